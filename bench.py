@@ -1,0 +1,291 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Langmuir hot path.
+
+Default workload (the configuration the headline metric is quoted on): the 3-D
+bulk-heterojunction solar cell, 128x128x32 lattice, two-phase morphology
+(uniform noise smoothed by a Gaussian of sigma = 4 sites, thresholded at its
+median; one phase carries 0.5 eV traps), 5242 electrons + 5242 holes (1 % of
+sites each) at uniformly random sites, exciton generation 0.524288 per step
+(1e-3 /site/ns at 1 ps steps over 128x128), Coulomb cutoff 30, drains 0.9 on
+both x faces.  One step = one KMC sweep (proposals, dE of every carrier, decisions,
+conflicts, removals, injection) on the device, with all state resident in HBM.
+
+Prints ONE JSON line.  ``value`` is KMC steps/s; ``pair_evals_per_s`` counts
+2 N (N + Nd) pair evaluations per step (the reference's all-pairs loop).
+``roofline`` is for the dominant kernel (de_kernel), FP32 VALU bound:
+11 flops per in-cutoff pair term, terms counted exactly on the timed state.
+``map`` times the full-lattice potential map (256x256x64, 50,000 charges).
+``cpu_baseline`` times the FP64 oracle (a restatement of the reference CPU path,
+OpenMP over carriers like QtConcurrent::map) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "KMC steps/s + Coulomb pair-evals/s; surrogate vs exact maps/s at ≤1e-3 err"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs
+# ---------------------------------------------------------------------------
+def morphology_traps(dims, sigma=4.0, seed=2024):
+    """Two-phase morphology: uniform noise in [-0.5, 0.5), Gaussian-smoothed
+    (sigma, truncated at 3 sigma), thresholded at its median; returns the site ids
+    of one phase (50 % of the lattice)."""
+    from scipy.ndimage import gaussian_filter
+    X, Y, Z = dims
+    rng = np.random.default_rng(seed)
+    noise = rng.random((Z, Y, X)) - 0.5
+    smooth = gaussian_filter(noise, sigma, mode="constant", truncate=3.0)
+    flat = smooth.ravel()
+    order = np.argsort(flat, kind="stable")
+    return np.sort(order[len(order) // 2:]).astype(np.int32)
+
+
+def solar_workload(seed=2024):
+    from langmuir_amd.host import SimulationParameters, background_potential
+    p = SimulationParameters(
+        simulation_type="solarcell", grid_x=128, grid_y=128, grid_z=32, coulomb_carriers=True,
+        electron_percentage=0.01, hole_percentage=0.01, seed_charges=1.0,
+        generation_rate=0.524288, source_scale_area=0.0, electrostatic_cutoff=30,
+        trap_potential=0.5, random_seed=12345)
+    V = p.volume
+    rng = np.random.default_rng(seed)
+    n = p.max_electrons()
+    e = rng.choice(V, n, replace=False).astype(np.int32)
+    h = rng.choice(V, n, replace=False).astype(np.int32)
+    traps = morphology_traps((128, 128, 32), seed=seed)
+    pot = background_potential(p, traps, np.full(len(traps), 0.5))
+    return p, e, h, traps, pot
+
+
+def transistor_workload(seed=2024):
+    from langmuir_amd.host import SimulationParameters
+    p = SimulationParameters(
+        grid_x=256, grid_y=256, grid_z=1, coulomb_carriers=True, electron_percentage=0.0,
+        hole_percentage=0.02, seed_charges=1.0, voltage_left=1.0, e_source_l_rate=0.0,
+        h_source_l_rate=0.9, h_drain_r_rate=0.9, random_seed=12345)
+    rng = np.random.default_rng(seed)
+    h = rng.choice(p.volume, p.max_holes(), replace=False).astype(np.int32)
+    return p, np.zeros(0, np.int32), h, np.zeros(0, np.int32), None
+
+
+# ---------------------------------------------------------------------------
+# helpers
+# ---------------------------------------------------------------------------
+def in_cutoff_terms(dims, cutoff, sites, fsites, charges):
+    """Exact count of (target, charge) pairs with 0 < r^2 < c^2 over the 2N targets."""
+    X, Y, Z = dims
+
+    def dec(s):
+        s = s.astype(np.int64)
+        z = s // (X * Y)
+        return np.stack([s % X, s // X - z * Y, z], 1).astype(np.int32)
+
+    c = dec(charges)
+    c2 = cutoff * cutoff
+    total = 0
+    for t in (dec(sites), dec(fsites)):
+        for a in range(0, len(t), 1024):
+            d = t[a:a + 1024, None, :] - c[None, :, :]
+            r2 = (d.astype(np.int32) ** 2).sum(-1)
+            total += int(((r2 > 0) & (r2 < c2)).sum())
+    return total
+
+
+class HipTimer:
+    """HIP events on an explicit stream (the context's), via libamdhip64."""
+
+    def __init__(self, stream):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.stream = C.c_void_p(stream)
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(C.byref(self.b)) == 0
+
+    def start(self):
+        assert self.hip.hipEventRecord(self.a, self.stream) == 0
+
+    def stop_ms(self):
+        assert self.hip.hipEventRecord(self.b, self.stream) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), self.a, self.b) == 0
+        return float(ms.value)
+
+
+def bench_map(reps=3):
+    """Full-lattice map, 256x256x64, 50,000 charges (+/-1 equal mix), cutoff 50 and none."""
+    import torch
+    import langmuir_amd as L
+    dims = (256, 256, 64)
+    V = 256 * 256 * 64
+    rng = np.random.default_rng(7)
+    n = 50000
+    cs = rng.choice(V, n, replace=False).astype(np.int32)
+    cq = np.where(np.arange(n) < n // 2, 1, -1).astype(np.int32)
+    out = {}
+    ctx = L.Context(dims, cutoff=50)
+    d_cs = torch.from_numpy(cs).cuda()
+    d_cq = torch.from_numpy(cq).cuda()
+    d_out = torch.empty(V, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    timer = HipTimer(ctx.stream)
+    for name, flags, cut in (("cutoff50", 0, 50), ("nocutoff", L.LMG_NO_CUTOFF, None)):
+        ctx.potential_map_dev(d_cs.data_ptr(), d_cq.data_ptr(), n, d_out.data_ptr(), flags)
+        ctx.sync()
+        timer.start()
+        for _ in range(reps):
+            ctx.potential_map_dev(d_cs.data_ptr(), d_cq.data_ptr(), n, d_out.data_ptr(), flags)
+        ms = timer.stop_ms() / reps
+        pair_evals = V * n
+        out[name] = dict(ms_per_map=round(ms, 4), maps_per_s=round(1e3 / ms, 3),
+                         pair_evals_per_s=pair_evals / (ms * 1e-3))
+        log(f"map {name}: {ms:.3f} ms/map, {pair_evals / (ms * 1e-3):.3e} pair-evals/s")
+    ctx.close()
+    return out
+
+
+def cpu_baseline(workload, budget_s=15.0):
+    """FP64 oracle (restated reference CPU path) on the same workload, all host cores."""
+    from oracle import pyoracle as O
+    p, e, h, traps, pot = workload
+    V = p.volume
+    kw = dict(simulation_type=1 if p.is_solarcell else 0, grid_x=p.grid_x, grid_y=p.grid_y,
+              grid_z=p.grid_z, coulomb_carriers=1, electron_percentage=p.electron_percentage,
+              hole_percentage=p.hole_percentage, seed_charges=0.0, generation_rate=p.generation_rate,
+              source_scale_area=p.source_scale_area, electrostatic_cutoff=p.electrostatic_cutoff,
+              voltage_left=p.voltage_left, voltage_right=p.voltage_right,
+              e_source_l_rate=p.e_source_l_rate, h_source_l_rate=p.h_source_l_rate,
+              h_drain_r_rate=p.h_drain_r_rate, random_seed=p.random_seed,
+              trap_percentage=(len(traps) + 0.5) / V if len(traps) else 0.0, trap_potential=0.5)
+    w = O.World(O.OrcParams.make(**kw), electrons=e, holes=h, traps=traps,
+                trap_potentials=np.full(len(traps), 0.5))
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    t0 = time.perf_counter()
+    w.step(1, threads)
+    per = time.perf_counter() - t0
+    steps = int(max(2, min(2000, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    w.step(steps, threads)
+    dt = time.perf_counter() - t0
+    n = len(e) + len(h)
+    return dict(value=round(steps / dt, 4), unit="KMC steps/s", cores=threads, kind="port",
+                sample=f"{steps} steps of the same workload from the same initial state "
+                       f"(N={n} carriers), FP64 oracle, OpenMP over carriers, {dt:.1f} s",
+                pair_evals_per_s=2.0 * n * n * steps / dt)
+
+
+# ---------------------------------------------------------------------------
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", choices=("solar", "transistor"), default="solar")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-map", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
+                                   "(replicas shard with no communication; see the design notes)"}))
+        sys.exit(2)
+
+    import langmuir_amd as L
+    workload = solar_workload() if args.workload == "solar" else transistor_workload()
+    p, e, h, traps, pot = workload
+    log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(e)} e + {len(h)} h, "
+        f"cutoff {p.electrostatic_cutoff}")
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot)
+    sim.step(args.warmup)
+    sim.ctx.sync()
+    # exact in-cutoff term count on the state entering the timed region
+    st = sim.state()
+    fe, fh, _, _ = sim.delta_e()
+    sites = np.concatenate([st["e_site"], st["h_site"]])
+    fsites = np.concatenate([fe, fh])
+    n_carriers = len(sites)
+    terms = in_cutoff_terms((p.grid_x, p.grid_y, p.grid_z), p.electrostatic_cutoff, sites, fsites,
+                            sites)
+    log(f"timed state: N={n_carriers}, in-cutoff terms per step {terms:.4e}")
+
+    t0 = time.perf_counter()
+    de_ms, commit_ms, span_ms = sim.step_timed(args.steps)
+    wall = time.perf_counter() - t0
+    assert sim.device_error() == 0
+    st2 = sim.state()
+    n_end = len(st2["e_site"]) + len(st2["h_site"])
+    steps_per_s = args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+    de_avg = de_ms / args.steps
+    flops = 11.0 * terms
+    achieved = flops / (de_avg * 1e-3) / 1e12
+    n_avg = 0.5 * (n_carriers + n_end)
+    pair_evals_per_s = 2.0 * n_avg * n_avg * steps_per_s
+    log(f"{steps_per_s:.1f} steps/s ({ms_per_step * 1e3:.1f} us/step); de_kernel {de_avg * 1e3:.1f} us, "
+        f"commit {commit_ms / args.steps * 1e3:.1f} us; {achieved:.2f} TFLOP/s")
+
+    res = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 3),
+        "unit": "KMC steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("solar BHJ 128x128x32, 5242 e + 5242 h seeded, two-phase sigma=4 morphology "
+                         "with 0.5 eV traps, generation 0.524288/step, cutoff 30"
+                         if args.workload == "solar" else
+                         "transistor 256x256x1, 1310 holes, 1 V bias, cutoff 50"),
+            "carriers_start": int(n_carriers), "carriers_end": int(n_end),
+            "parallelism": "single GPU",
+        },
+        "pair_evals_per_s": pair_evals_per_s,
+        "kernel_us": {"de_kernel": round(de_avg * 1e3, 3),
+                      "commit_kernel": round(commit_ms / args.steps * 1e3, 3),
+                      "event_span_per_step": round(span_ms / args.steps * 1e3, 3)},
+        "roofline": {"bound": "valu", "kernel": "de_kernel", "achieved": round(achieved, 4),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                     "per_launch": f"11 flops x {terms} in-cutoff terms"},
+    }
+    if not args.no_map:
+        res["map"] = bench_map()
+    res["surrogate"] = None
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(workload, args.cpu_budget)
+        res["cpu_baseline"] = cb
+        res["speedup_vs_cpu"] = round(steps_per_s / cb["value"], 2)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+/*
+ * langmuir_gpu.h -- C-ABI of the MI355X (gfx950) implementation of Langmuir's
+ * data-parallel hot path.
+ *
+ * It replaces the C++ class LangmuirCore::OpenClHelper
+ * (src/langmuirCore/include/openclhelper.h:22-241, openclhelper.cpp:17-583)
+ * and the per-step sweep it served (src/langmuirCore/simulation.cpp:29-111).
+ * Plain pointers and sizes only; no C++ types, no exceptions and no abort
+ * cross this boundary.  Every function returns an lmg_status; the message of
+ * the last failure is lmg_last_error(ctx) (ctx may be NULL for failures of
+ * lmg_create).
+ *
+ * Threading: a context is single-threaded (callers serialise); distinct
+ * contexts are independent.  Ownership: the caller owns every host array for
+ * the duration of a call; the context owns all device memory.  Calls taking
+ * host pointers are synchronous; the *_dev variants take device pointers, are
+ * ordered on the context's stream and return before the device finishes
+ * (lmg_sync waits).
+ *
+ * Site ids follow the reference: s = X*(y + z*Y) + x
+ * (src/langmuirCore/cubicgrid.cpp:162-165); ids V..V+34 are the special-agent
+ * slots of a grid (cubicgrid.cpp:558-589).
+ */
+#ifndef LANGMUIR_GPU_H
+#define LANGMUIR_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lmg_status;
+#define LMG_OK             0
+#define LMG_ERR_ARG       -1   /* invalid argument or size                        */
+#define LMG_ERR_NO_DEVICE -2   /* no usable GPU: the OpenClHelper okCL=false case  */
+#define LMG_ERR_HIP       -3   /* HIP runtime failure                              */
+#define LMG_ERR_CAPACITY  -4   /* input larger than the context was sized for     */
+#define LMG_ERR_STATE     -5   /* call out of order (e.g. step before load)       */
+#define LMG_ERR_DEVICE    -6   /* device-side consistency check failed            */
+
+/* pair-sum flags */
+#define LMG_GAUSS      1  /* terms times erf(r / (sqrt(2) sigma)): gauss1/gauss2        */
+#define LMG_IMAGE_LEFT 2  /* left-electrode image, dx = |xi + xj| + 1, sign -1
+                             (Potential::coulombImage*, potential.cpp:384-415)          */
+#define LMG_NO_CUTOFF  4  /* ignore the cutoff (map only)                               */
+
+typedef struct lmg_ctx lmg_ctx;
+
+/* Constant inputs of the hot path; the role of the World/SimulationParameters
+ * fields OpenClHelper::initializeOpenCL pre-binds (openclhelper.cpp:124-168). */
+typedef struct lmg_params {
+    int32_t grid_x, grid_y, grid_z;  /* grid.x/y/z                                 */
+    int32_t cutoff;                  /* electrostatic.cutoff (sites)               */
+    int32_t hopping_range;           /* hopping.range, 1 or 2                      */
+    int32_t defects_charge;          /* defects.charge                             */
+    int32_t device_id;               /* opencl.device.id analogue (HIP ordinal)    */
+    int32_t reserved;
+    double prefactor;                /* electrostatic.prefactor, parameters.h:386  */
+    double inverse_kt;               /* inverse.kt, parameters.h:387               */
+    double gaussian_sigma;           /* coulomb.gaussian.sigma (0 = plain 1/r)     */
+    double exciton_binding;          /* exciton.binding                            */
+} lmg_params;
+
+/* Create / destroy: replaces initializeOpenCL + toggleOpenCL
+ * (openclhelper.cpp:17-218).  Fails with LMG_ERR_NO_DEVICE when no GPU is
+ * present, which a host maps onto the reference's CPU fallback rule. */
+lmg_status lmg_create(const lmg_params* params, lmg_ctx** out);
+void lmg_destroy(lmg_ctx* ctx);
+const char* lmg_last_error(const lmg_ctx* ctx);
+int32_t lmg_device_count(void);
+lmg_status lmg_sync(lmg_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own work. */
+void* lmg_stream(lmg_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Pair sums at an explicit target list: the coulomb2/gauss2 kernels with
+ * launchCoulombKernel2/launchGaussKernel2 + getOutputHost
+ * (kernel.cl:222-369, openclhelper.cpp:372-544,567-583), and the image sums
+ * Potential::coulombImageE/H/D.  out[t] = pref * sum_j q_j f(r_tj) over
+ * 0 < r^2 < cutoff^2 (times -1 for LMG_IMAGE_LEFT), FP32 terms with
+ * compensated accumulation, delivered as double.
+ * ------------------------------------------------------------------------- */
+lmg_status lmg_pair_potential(lmg_ctx* ctx, const int32_t* charge_site,
+                              const int32_t* charge_q, int32_t n_charges,
+                              const int32_t* target_site, int32_t n_targets,
+                              int32_t flags, double* out);
+lmg_status lmg_pair_potential_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
+                                  const int32_t* d_charge_q, int32_t n_charges,
+                                  const int32_t* d_target_site, int32_t n_targets,
+                                  int32_t flags, double* d_out);
+
+/* ---------------------------------------------------------------------------
+ * Full-lattice potential map: the coulomb1/gauss1 kernels with
+ * launchCoulombKernel1 (kernel.cl:56-220, openclhelper.cpp:220-370).  Writes
+ * all V sites (the reference read back only n values).  out is float[V].
+ * ------------------------------------------------------------------------- */
+lmg_status lmg_potential_map(lmg_ctx* ctx, const int32_t* charge_site,
+                             const int32_t* charge_q, int32_t n_charges,
+                             int32_t flags, float* out);
+lmg_status lmg_potential_map_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
+                                 const int32_t* d_charge_q, int32_t n_charges,
+                                 int32_t flags, float* d_out);
+
+/* ---------------------------------------------------------------------------
+ * KMC sweep on the device (Simulation::performIterations with the Coulomb
+ * branch, simulation.cpp:29-111, and the agents it drives).
+ * ------------------------------------------------------------------------- */
+
+/* flux agents, World order (world.cpp:961-1122) */
+#define LMG_FLUX_ESL 0
+#define LMG_FLUX_ESR 1
+#define LMG_FLUX_HSL 2
+#define LMG_FLUX_HSR 3
+#define LMG_FLUX_XS  4
+#define LMG_FLUX_EDL 5
+#define LMG_FLUX_EDR 6
+#define LMG_FLUX_HDL 7
+#define LMG_FLUX_HDR 8
+#define LMG_FLUX_XD  9
+
+typedef struct lmg_sim_config {
+    int32_t simulation_type;        /* 0 transistor, 1 solarcell                   */
+    int32_t coulomb_carriers;       /* coulomb.carriers                            */
+    int32_t source_metropolis;      /* source.metropolis                           */
+    int32_t source_coulomb;         /* source.coulomb                              */
+    int32_t balance_charges;        /* balance.charges                             */
+    int32_t recombination_range;    /* recombination.range                         */
+    int32_t max_electrons;          /* World::maxElectronAgents (world.cpp:598)    */
+    int32_t max_holes;              /* World::maxHoleAgents (world.cpp:595)        */
+    int32_t e_drain_slot[2];        /* slot ids of the left/right electron drains  */
+    int32_t h_drain_slot[2];        /* slot ids of the left/right hole drains      */
+    double voltage_left, voltage_right;
+    double rate[10];                /* flux-agent probabilities (m_probability)     */
+    double coupling[27];            /* couplingConstants, (h+1)^3, [dx][dy][dz]    */
+} lmg_sim_config;
+
+/* Host-side state image.  For lmg_sim_load every array holds n_* entries; for
+ * lmg_sim_store the caller provides arrays of at least the capacities
+ * (max_electrons, max_holes) and the callee sets the n_* fields. */
+typedef struct lmg_sim_state {
+    int32_t n_electrons, n_holes, n_defects, reserved;
+    int32_t* e_site;  int32_t* e_lifetime;  int32_t* e_pathlength;
+    int32_t* h_site;  int32_t* h_lifetime;  int32_t* h_pathlength;
+    int32_t* defect_site;           /* load only                                   */
+    double* potential;              /* load only: background potential, V sites    */
+    uint32_t rng_window[624];       /* raw MT19937 words x[k .. k+623]             */
+    uint64_t rng_consumed;          /* k = engine outputs consumed since seeding    */
+    uint64_t flux_attempts[10];
+    uint64_t flux_successes[10];
+    int64_t current_step;
+} lmg_sim_state;
+
+lmg_status lmg_sim_load(lmg_ctx* ctx, const lmg_sim_config* config,
+                        const lmg_sim_state* state);
+lmg_status lmg_sim_store(lmg_ctx* ctx, lmg_sim_state* state);
+/* n KMC steps, enqueued on the context stream; returns once enqueued unless
+ * the context is in synchronous mode (lmg_sim_set_sync). */
+lmg_status lmg_sim_step(lmg_ctx* ctx, int32_t n_steps);
+lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
+/* lmg_sim_step with HIP events around every kernel on the context stream;
+ * returns once the steps are done.  kernel_ms[0] = summed dE-kernel time,
+ * kernel_ms[1] = summed commit-kernel time, kernel_ms[2] = first-to-last
+ * event span (no counterpart in the reference: its only timing is the
+ * real:time column of out.dat, writer.cpp:362,376). */
+lmg_status lmg_sim_step_timed(lmg_ctx* ctx, int32_t n_steps, double* kernel_ms);
+/* Proposed sites and dE of the pending step for every carrier (electrons then
+ * holes) without consuming the decision draws: the values
+ * ChargeAgent::coulombCPU/coulombGPU would hold (chargeagent.cpp:215-296). */
+lmg_status lmg_sim_delta_e(lmg_ctx* ctx, int32_t* fsite_e, int32_t* fsite_h,
+                           double* de_e, double* de_h);
+/* *code = 0 when healthy; otherwise a device-side error code:
+ * 1 = random-word ring overrun, 2 = carrier list above capacity,
+ * 3 = invalid site in a carrier list, 4 = balanceCharges gave up (the
+ * reference's qFatal "can not balance"). */
+lmg_status lmg_sim_device_error(lmg_ctx* ctx, int32_t* code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LANGMUIR_GPU_H */

@@ -1,0 +1,117 @@
+"""ctypes declarations of the C-ABI in include/langmuir_gpu.h.
+
+The shared library ``liblangmuir_gpu.so`` is built in-tree next to this file by
+``langmuir_amd.build``.  Loading fails loudly when it is missing: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblangmuir_gpu.so")
+
+I32 = C.c_int32
+U32 = C.c_uint32
+U64 = C.c_uint64
+I64 = C.c_int64
+F64 = C.c_double
+VP = C.c_void_p
+PI32 = C.POINTER(C.c_int32)
+PF64 = C.POINTER(C.c_double)
+
+LMG_OK = 0
+LMG_ERR_ARG = -1
+LMG_ERR_NO_DEVICE = -2
+LMG_ERR_HIP = -3
+LMG_ERR_CAPACITY = -4
+LMG_ERR_STATE = -5
+LMG_ERR_DEVICE = -6
+
+LMG_GAUSS = 1
+LMG_IMAGE_LEFT = 2
+LMG_NO_CUTOFF = 4
+
+FLUX_NAMES = ("eSL", "eSR", "hSL", "hSR", "xS", "eDL", "eDR", "hDL", "hDR", "xD")
+
+
+class LmgParams(C.Structure):
+    _fields_ = [
+        ("grid_x", I32), ("grid_y", I32), ("grid_z", I32), ("cutoff", I32),
+        ("hopping_range", I32), ("defects_charge", I32), ("device_id", I32), ("reserved", I32),
+        ("prefactor", F64), ("inverse_kt", F64), ("gaussian_sigma", F64),
+        ("exciton_binding", F64),
+    ]
+
+
+class LmgSimConfig(C.Structure):
+    _fields_ = [
+        ("simulation_type", I32), ("coulomb_carriers", I32), ("source_metropolis", I32),
+        ("source_coulomb", I32), ("balance_charges", I32), ("recombination_range", I32),
+        ("max_electrons", I32), ("max_holes", I32), ("e_drain_slot", I32 * 2),
+        ("h_drain_slot", I32 * 2), ("voltage_left", F64), ("voltage_right", F64),
+        ("rate", F64 * 10), ("coupling", F64 * 27),
+    ]
+
+
+class LmgSimState(C.Structure):
+    _fields_ = [
+        ("n_electrons", I32), ("n_holes", I32), ("n_defects", I32), ("reserved", I32),
+        ("e_site", PI32), ("e_lifetime", PI32), ("e_pathlength", PI32),
+        ("h_site", PI32), ("h_lifetime", PI32), ("h_pathlength", PI32),
+        ("defect_site", PI32), ("potential", PF64),
+        ("rng_window", U32 * 624), ("rng_consumed", U64),
+        ("flux_attempts", U64 * 10), ("flux_successes", U64 * 10), ("current_step", I64),
+    ]
+
+
+class LmgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"langmuir_gpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load liblangmuir_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import langmuir_amd.build as b; b.build()'`"
+            " (there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    L.lmg_last_error.restype = C.c_char_p
+    L.lmg_last_error.argtypes = [VP]
+    L.lmg_device_count.restype = I32
+    L.lmg_create.argtypes = [C.POINTER(LmgParams), C.POINTER(VP)]
+    L.lmg_destroy.argtypes = [VP]
+    L.lmg_destroy.restype = None
+    L.lmg_sync.argtypes = [VP]
+    L.lmg_stream.argtypes = [VP]
+    L.lmg_stream.restype = VP
+    L.lmg_pair_potential.argtypes = [VP, VP, VP, I32, VP, I32, I32, VP]
+    L.lmg_pair_potential_dev.argtypes = [VP, VP, VP, I32, VP, I32, I32, VP]
+    L.lmg_potential_map.argtypes = [VP, VP, VP, I32, I32, VP]
+    L.lmg_potential_map_dev.argtypes = [VP, VP, VP, I32, I32, VP]
+    L.lmg_sim_load.argtypes = [VP, C.POINTER(LmgSimConfig), C.POINTER(LmgSimState)]
+    L.lmg_sim_store.argtypes = [VP, C.POINTER(LmgSimState)]
+    L.lmg_sim_step.argtypes = [VP, I32]
+    L.lmg_sim_set_sync.argtypes = [VP, I32]
+    L.lmg_sim_delta_e.argtypes = [VP, VP, VP, VP, VP]
+    L.lmg_sim_device_error.argtypes = [VP, PI32]
+    L.lmg_sim_step_timed.argtypes = [VP, I32, PF64]
+    _lib = L
+    return L
+
+
+EXPORTED = (
+    "lmg_create", "lmg_destroy", "lmg_last_error", "lmg_device_count", "lmg_sync", "lmg_stream",
+    "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_potential_map", "lmg_potential_map_dev",
+    "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_delta_e",
+    "lmg_sim_device_error", "lmg_sim_step_timed",
+)

@@ -1,0 +1,164 @@
+// lmg_device.h -- device-side building blocks shared by the Langmuir HIP kernels
+// (gfx950, wave64).  Lattice indexing, the MT19937 stream, Boost's uniform_int
+// bucket rule and the neighbour order of the reference grid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lmg {
+
+constexpr int kSpecialSlots = 35;   // Grid::m_specialAgentReserve (cubicgrid.cpp:22)
+
+// Agent::Type (include/agent.h:33-55)
+constexpr uint8_t kEmpty = 0, kElectron = 1, kHole = 2, kDefect = 3, kSource = 4, kDrain = 5;
+
+struct Dims {
+    int X, Y, Z, XY, V;
+};
+
+// cubicgrid.cpp:132-145: x = s % X, y = s / X - (s / XY) * Y, z = s / XY.  Also
+// used on special slots s >= V, exactly like kernel.cl:229-231.
+__host__ __device__ inline void decode(const Dims& d, int s, int& x, int& y, int& z) {
+    z = s / d.XY;
+    y = s / d.X - z * d.Y;
+    x = s % d.X;
+}
+__host__ __device__ inline int encode(const Dims& d, int x, int y, int z) {
+    return d.X * (y + z * d.Y) + x;   // cubicgrid.cpp:162-165
+}
+
+// ---------------------------------------------------------------------------
+// MT19937 as a word sequence: x[n+624] = x[n+397] ^ twist(x[n], x[n+1]);
+// engine output k (k = 0, 1, ...) is temper(x[k + 624]).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t mt_temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+__host__ __device__ inline uint32_t mt_next(uint32_t x0, uint32_t x1, uint32_t x397) {
+    uint32_t y = (x0 & 0x80000000u) | (x1 & 0x7fffffffu);
+    return x397 ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+
+// uniform_01<double> on a 32-bit engine: u32 * 2^-32 (rand.cpp:49-52)
+__host__ __device__ inline double u01(uint32_t w) { return double(w) * (1.0 / 4294967296.0); }
+
+// Boost >= 1.47 uniform_int bucket size for range = hi - lo in [1, 2^32 - 2]
+__host__ __device__ inline uint32_t uint_bucket(uint32_t range) {
+    uint32_t b = 0xffffffffu / (range + 1u);
+    if (0xffffffffu % (range + 1u) == range) ++b;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists (Grid::neighborsSite, cubicgrid.cpp:207-471).  The order is
+// part of the contract: it is indexed by the proposal draw.
+// ---------------------------------------------------------------------------
+struct Coord {
+    int x, y, z;
+};
+
+// hopping.range = 1 without building a list: count, and the k-th entry.
+__device__ inline int nbr1_count(const Dims& d, int x, int y, int z, int dl, int dr) {
+    int n = (x > 0) + (x < d.X - 1) + (y > 0) + (y < d.Y - 1) + (z > 0) + (z < d.Z - 1);
+    n += (x == 0 && dl >= 0) + (x == d.X - 1 && dr >= 0);
+    return n;
+}
+__device__ inline int nbr1_at(const Dims& d, int x, int y, int z, int dl, int dr, int k) {
+    // candidates in reference order: W E S N down up drainL drainR
+    if (x > 0) { if (k == 0) return encode(d, x - 1, y, z); --k; }
+    if (x < d.X - 1) { if (k == 0) return encode(d, x + 1, y, z); --k; }
+    if (y > 0) { if (k == 0) return encode(d, x, y - 1, z); --k; }
+    if (y < d.Y - 1) { if (k == 0) return encode(d, x, y + 1, z); --k; }
+    if (z > 0) { if (k == 0) return encode(d, x, y, z - 1); --k; }
+    if (z < d.Z - 1) { if (k == 0) return encode(d, x, y, z + 1); --k; }
+    if (x == 0 && dl >= 0) { if (k == 0) return dl; --k; }
+    return dr;
+}
+
+// General list for hopping.range 1 or 2; out needs 34 entries.
+__device__ inline int nbr_list(const Dims& d, int h, int site, int dl, int dr, int* out) {
+    int x, y, z;
+    decode(d, site, x, y, z);
+    int n = 0;
+#define LMG_P(a, b, c) (out[n++] = encode(d, (a), (b), (c)))
+    if (h == 1) {
+        if (x > 0) LMG_P(x - 1, y, z);
+        if (x < d.X - 1) LMG_P(x + 1, y, z);
+        if (y > 0) LMG_P(x, y - 1, z);
+        if (y < d.Y - 1) LMG_P(x, y + 1, z);
+        if (z > 0) LMG_P(x, y, z - 1);
+        if (z < d.Z - 1) LMG_P(x, y, z + 1);
+    } else if (d.Z == 1) {
+        if (x > 0) {
+            LMG_P(x - 1, y, z);
+            if (y > 0) LMG_P(x - 1, y - 1, z);
+            if (y < d.Y - 1) LMG_P(x - 1, y + 1, z);
+            if (x > 1) LMG_P(x - 2, y, z);
+        }
+        if (x < d.X - 1) {
+            LMG_P(x + 1, y, z);
+            if (y > 0) LMG_P(x + 1, y - 1, z);
+            if (y < d.Y - 1) LMG_P(x + 1, y + 1, z);
+            if (x < d.X - 2) LMG_P(x + 2, y, z);
+        }
+        if (y > 0) {
+            LMG_P(x, y - 1, z);
+            if (y > 1) LMG_P(x, y - 2, z);
+        }
+        if (y < d.Y - 1) {
+            LMG_P(x, y + 1, z);
+            if (y < d.Y - 2) LMG_P(x, y + 2, z);
+        }
+    } else {
+        for (int side = -1; side <= 1; side += 2) {
+            bool ok = side < 0 ? (x > 0) : (x < d.X - 1);
+            if (!ok) continue;
+            int xx = x + side;
+            LMG_P(xx, y, z);
+            if (y > 0) {
+                LMG_P(xx, y - 1, z);
+                if (z > 0) LMG_P(xx, y - 1, z - 1);
+                if (z < d.Z - 1) LMG_P(xx, y - 1, z + 1);
+            }
+            if (y < d.Y - 1) {
+                LMG_P(xx, y + 1, z);
+                if (z > 0) LMG_P(xx, y + 1, z - 1);
+                if (z < d.Z - 1) LMG_P(xx, y + 1, z + 1);
+            }
+            if (z > 0) LMG_P(xx, y, z - 1);
+            if (z < d.Z - 1) LMG_P(xx, y, z + 1);
+            if (side < 0 ? (x > 1) : (x < d.X - 2)) LMG_P(x + 2 * side, y, z);
+        }
+        if (z > 0) {
+            LMG_P(x, y, z - 1);
+            if (y > 0) LMG_P(x, y - 1, z - 1);
+            if (y < d.Y - 1) LMG_P(x, y + 1, z - 1);
+            if (z > 1) LMG_P(x, y, z - 2);
+        }
+        if (z < d.Z - 1) {
+            LMG_P(x, y, z + 1);
+            if (y > 0) LMG_P(x, y - 1, z + 1);
+            if (y < d.Y - 1) LMG_P(x, y + 1, z + 1);
+            if (z < d.Z - 2) LMG_P(x, y, z + 2);
+        }
+        if (y > 0) {
+            LMG_P(x, y - 1, z);
+            if (y > 1) LMG_P(x, y - 2, z);
+        }
+        if (y < d.Y - 1) {
+            LMG_P(x, y + 1, z);
+            if (y < d.Y - 2) LMG_P(x, y + 2, z);
+        }
+    }
+#undef LMG_P
+    if (x == 0 && dl >= 0) out[n++] = dl;        // cubicgrid.cpp:447-457
+    if (x == d.X - 1 && dr >= 0) out[n++] = dr;  // cubicgrid.cpp:459-468
+    return n;
+}
+
+}  // namespace lmg

@@ -1,0 +1,1178 @@
+/*
+ * lmoracle.cpp -- CPU FP64 ORACLE of the Langmuir KMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker (and the timed CPU
+ * baseline) for the HIP implementation in langmuir_amd/.  It is loaded only by
+ * tests/, by the smoke check and by the CPU-baseline leg of the benchmark.  The
+ * product library never links or calls it.
+ *
+ * It restates, function by function, the reference CPU path (paths relative to
+ * the reference root, src/langmuirCore/):
+ *   rand.cpp:8-124             Random: boost::mt19937, uniform_01, uniform_int,
+ *                              metropolisWithCoupling, chooseYes
+ *   cubicgrid.cpp:9-33,102-165 Grid sizes, index maps, |d| and image distances
+ *   cubicgrid.cpp:207-471      neighborsSite (order matters: it is RNG order)
+ *   cubicgrid.cpp:473-526      neighborsFace (sorted face slices)
+ *   cubicgrid.cpp:558-636      special-agent slots V+k and agent registration
+ *   potential.cpp:17-244       background potential (zero, linear, gate, traps)
+ *   potential.cpp:246-349      precalculateArrays, updateCouplingConstants
+ *   potential.cpp:351-753      coulombE/H/D, coulombImageE/H/D, gaussE/H/D
+ *   chargeagent.cpp:73-296     chooseFuture, decideFuture, completeTick, coulombCPU
+ *   chargeagent.cpp:463-481    bindingPotential
+ *   simulation.cpp:29-414      performIterations, recombination, injection,
+ *                              balanceCharges, nextTick
+ *   sourceagent.cpp:71-247     tryToSeed/tryToInject, chooseSite, energyChange,
+ *                              shouldTransport, validToInject, inject
+ *   drainagent.cpp:83-214      DrainAgent/RecombinationAgent::tryToAccept
+ *   fluxagent.cpp:125-130      FluxAgent::shouldTransport = chooseYes(rate)
+ *   world.cpp:501-1187         construction order (RNG order and slot ids)
+ *   include/parameters.h:384-390  setCalculatedValues
+ *
+ * Boost is not part of the reference tree; its algorithms are restated from the
+ * published Boost.Random sources (>= 1.47): mt19937 (the C++11 engine),
+ * uniform_01<double> = u32 * 2^-32, uniform_int = bucket rejection.
+ * boost::normal_distribution is not restated (its algorithm changed between
+ * Boost versions); a world that would draw from it is refused.
+ */
+#include "lmoracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+
+struct OracleError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+[[noreturn]] void fail(const std::string& m) { throw OracleError(m); }
+
+/* ------------------------------------------------------------------------- */
+/* MT19937 in sequence form: x[n+624] = x[n+397] ^ twist(x[n], x[n+1]).      */
+/* The window holds x[k .. k+623] where k = outputs consumed since seeding;  */
+/* the next output is temper(x[k+624]).  Seeding is boost/std::mt19937's.     */
+/* ------------------------------------------------------------------------- */
+struct MT19937 {
+    uint32_t w[624];
+    int head = 0;          // x[k+i] = w[(head+i) % 624]
+    uint64_t consumed = 0; // k
+
+    void seed(uint32_t s) {
+        w[0] = s;
+        for (int i = 1; i < 624; ++i)
+            w[i] = 1812433253u * (w[i - 1] ^ (w[i - 1] >> 30)) + uint32_t(i);
+        head = 0;
+        consumed = 0;
+    }
+    static uint32_t temper(uint32_t y) {
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    uint32_t next_raw() {
+        int h1 = head + 1; if (h1 == 624) h1 = 0;
+        int h397 = head + 397; if (h397 >= 624) h397 -= 624;
+        uint32_t y = (w[head] & 0x80000000u) | (w[h1] & 0x7fffffffu);
+        uint32_t v = w[h397] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        w[head] = v;
+        head = h1;
+        ++consumed;
+        return v;
+    }
+    uint32_t operator()() { return temper(next_raw()); }
+    void get_window(uint32_t* out) const {
+        for (int i = 0; i < 624; ++i) out[i] = w[(head + i) % 624];
+    }
+    void set_window(const uint32_t* in, uint64_t k) {
+        std::memcpy(w, in, sizeof(w));
+        head = 0;
+        consumed = k;
+    }
+};
+
+/* Boost >= 1.47 generate_uniform_int for a 32-bit engine with brange = 2^32-1. */
+template <class Eng>
+int uniform_int(Eng& eng, int lo, int hi) {
+    if (hi < lo) fail("uniform_int: empty range");
+    const uint32_t range = uint32_t(hi) - uint32_t(lo);
+    if (range == 0) return lo;                 // no engine call
+    const uint32_t brange = 0xffffffffu;
+    if (range == brange) return int(uint32_t(lo) + eng());
+    uint32_t bucket = brange / (range + 1u);
+    if (brange % (range + 1u) == range) ++bucket;
+    for (;;) {
+        uint32_t r = eng() / bucket;
+        if (r <= range) return int(uint32_t(lo) + r);
+    }
+}
+
+/* rand.cpp:8-124 */
+struct Random {
+    MT19937 mt;
+    double random() { return double(mt()) * (1.0 / 4294967296.0); }     // :49-52
+    int integer(int lo, int hi) { return uniform_int(mt, lo, hi); }     // :68-73
+    // :91-106 ; also returns the draw and the margin for parity classification
+    bool metropolisWithCoupling(double dE, double beta, double coupling,
+                                double* u_out = nullptr, double* m_out = nullptr) {
+        double u = random();
+        double lhs = (dE > 0.0) ? coupling * std::exp(-dE * beta) : coupling;
+        if (u_out) *u_out = u;
+        if (m_out) *m_out = lhs - u;
+        return lhs > u;
+    }
+    bool chooseYes(double p, double* u_out = nullptr, double* m_out = nullptr) {  // :117-124
+        double u = random();
+        if (u_out) *u_out = u;
+        if (m_out) *m_out = p - u;
+        return p > u;
+    }
+};
+
+/* include/agent.h:33-55 */
+enum AgentType : uint8_t { Empty = 0, Electron = 1, Hole = 2, Defect = 3, Source = 4, Drain = 5 };
+/* include/cubicgrid.h:46-70 */
+enum Face { Left = 0, Right = 1, Top = 2, Bottom = 3, Front = 4, Back = 5, NoFace = 6 };
+/* flux agents in World order (world.cpp:961-1122): m_fluxAgents */
+enum Flux { ESL = 0, ESR = 1, HSL = 2, HSR = 3, XS = 4, EDL = 5, EDR = 6, HDL = 7, HDR = 8, XD = 9 };
+
+struct Carrier;
+
+/* cubicgrid.cpp */
+struct Grid {
+    int X = 0, Y = 0, Z = 0, XY = 0, V = 0;
+    int special_count = 0;
+    static constexpr int kSpecialReserve = 35;               // :22
+    std::vector<uint8_t> type;     // V + 35
+    std::vector<Carrier*> at;      // charge agent address at a lattice site
+    std::vector<int> special_flux; // flux index of the agent in slot V+k
+    std::vector<double> pot;       // V + 35
+    std::vector<std::pair<int, uint8_t>> face_agents[7];   // (slot, type)
+
+    void init(int x, int y, int z) {
+        X = x; Y = y; Z = z; XY = x * y; V = x * y * z;
+        type.assign(V + kSpecialReserve, Empty);
+        at.assign(V + kSpecialReserve, nullptr);
+        special_flux.assign(kSpecialReserve, -1);
+        pot.assign(V + kSpecialReserve, 0.0);
+    }
+    int gx(int s) const { return s % X; }                    // :132-135
+    int gy(int s) const { return s / X - (s / XY) * Y; }     // :137-140
+    int gz(int s) const { return s / XY; }                   // :142-145
+    int idx(int x, int y, int z) const { return X * (y + z * Y) + x; }   // :162-165
+    int dxi(int a, int b) const { return std::abs(gx(a) - gx(b)); }      // :102-115
+    int dyi(int a, int b) const { return std::abs(gy(a) - gy(b)); }
+    int dzi(int a, int b) const { return std::abs(gz(a) - gz(b)); }
+    int dxi_image(int a, int b) const { return std::abs(gx(a) + gx(b)) + 1; }  // :117-120
+
+    /* :558-589 */
+    int register_special(uint8_t t, int face, int flux) {
+        if (special_count >= kSpecialReserve) fail("special agent reserve exceeded");
+        int slot = V + special_count;
+        if (type[slot] != Empty) fail("special slot occupied");
+        type[slot] = t;
+        special_flux[special_count] = flux;
+        face_agents[face].push_back({slot, t});
+        ++special_count;
+        return slot;
+    }
+
+    /* :473-526 (sliceIndex sorts the result) */
+    std::vector<int> neighbors_face(int face) const {
+        std::vector<int> v;
+        auto slice = [&](int xi, int xf, int yi, int yf, int zi, int zf) {
+            for (int i = xi; i < xf; ++i)
+                for (int j = yi; j < yf; ++j)
+                    for (int k = zi; k < zf; ++k) v.push_back(idx(i, j, k));
+            std::sort(v.begin(), v.end());
+        };
+        switch (face) {
+            case Left:  slice(0, 1, 0, Y, 0, Z); break;
+            case Right: slice(X - 1, X, 0, Y, 0, Z); break;
+            default: break;   // only the x faces carry sources in this path
+        }
+        return v;
+    }
+
+    /* :207-471 -- order is significant (it is the RNG index order) */
+    std::vector<int> neighbors(int site, int h) const {
+        std::vector<int> n;
+        n.reserve(40);
+        const int x = gx(site), y = gy(site), z = gz(site);
+        auto P = [&](int a, int b, int c) { n.push_back(idx(a, b, c)); };
+        if (h == 1) {
+            if (x > 0) P(x - 1, y, z);
+            if (x < X - 1) P(x + 1, y, z);
+            if (y > 0) P(x, y - 1, z);
+            if (y < Y - 1) P(x, y + 1, z);
+            if (z > 0) P(x, y, z - 1);
+            if (z < Z - 1) P(x, y, z + 1);
+        } else if (h == 2) {
+            if (Z == 1) {
+                if (x > 0) {
+                    P(x - 1, y, z);
+                    if (y > 0) P(x - 1, y - 1, z);
+                    if (y < Y - 1) P(x - 1, y + 1, z);
+                    if (x > 1) P(x - 2, y, z);
+                }
+                if (x < X - 1) {
+                    P(x + 1, y, z);
+                    if (y > 0) P(x + 1, y - 1, z);
+                    if (y < Y - 1) P(x + 1, y + 1, z);
+                    if (x < X - 2) P(x + 2, y, z);
+                }
+                if (y > 0) {
+                    P(x, y - 1, z);
+                    if (y > 1) P(x, y - 2, z);
+                }
+                if (y < Y - 1) {
+                    P(x, y + 1, z);
+                    if (y < Y - 2) P(x, y + 2, z);
+                }
+            } else {
+                for (int side = -1; side <= 1; side += 2) {
+                    bool ok = side < 0 ? (x > 0) : (x < X - 1);
+                    if (!ok) continue;
+                    int xx = x + side;
+                    P(xx, y, z);
+                    if (y > 0) {
+                        P(xx, y - 1, z);
+                        if (z > 0) P(xx, y - 1, z - 1);
+                        if (z < Z - 1) P(xx, y - 1, z + 1);
+                    }
+                    if (y < Y - 1) {
+                        P(xx, y + 1, z);
+                        if (z > 0) P(xx, y + 1, z - 1);
+                        if (z < Z - 1) P(xx, y + 1, z + 1);
+                    }
+                    if (z > 0) P(xx, y, z - 1);
+                    if (z < Z - 1) P(xx, y, z + 1);
+                    if (side < 0 ? (x > 1) : (x < X - 2)) P(x + 2 * side, y, z);
+                }
+                if (z > 0) {
+                    P(x, y, z - 1);
+                    if (y > 0) P(x, y - 1, z - 1);
+                    if (y < Y - 1) P(x, y + 1, z - 1);
+                    if (z > 1) P(x, y, z - 2);
+                }
+                if (z < Z - 1) {
+                    P(x, y, z + 1);
+                    if (y > 0) P(x, y - 1, z + 1);
+                    if (y < Y - 1) P(x, y + 1, z + 1);
+                    if (z < Z - 2) P(x, y, z + 2);
+                }
+                if (y > 0) {
+                    P(x, y - 1, z);
+                    if (y > 1) P(x, y - 2, z);
+                }
+                if (y < Y - 1) {
+                    P(x, y + 1, z);
+                    if (y < Y - 2) P(x, y + 2, z);
+                }
+            }
+        } else {
+            fail("invalid hopping range");
+        }
+        // :447-468 drains appended on the x faces, in registration order
+        if (x == 0)
+            for (auto& a : face_agents[Left]) if (a.second == Drain) n.push_back(a.first);
+        if (x == X - 1)
+            for (auto& a : face_agents[Right]) if (a.second == Drain) n.push_back(a.first);
+        return n;
+    }
+};
+
+struct Params {
+    orc_params p;
+    double prefactor = 0, inverse_kt = 0;
+};
+
+struct Carrier {
+    int q = 0;          // -1 electron, +1 hole
+    int site = 0, fsite = 0;
+    int lifetime = 0, pathlength = 0;
+    double de = 0;
+    bool removed = false;
+    std::vector<int> nbrs;
+};
+
+/* 3-D table with the reference's [dx][dy][dz] indexing */
+struct Table3 {
+    int n = 0;
+    std::vector<double> v;
+    void resize(int m) { n = m; v.assign(size_t(m) * m * m, 0.0); }
+    double& operator()(int a, int b, int c) { return v[(size_t(a) * n + b) * n + c]; }
+    double operator()(int a, int b, int c) const { return v[(size_t(a) * n + b) * n + c]; }
+};
+
+}  // namespace
+
+struct orc_world {
+    Params P;
+    Random rng;
+    Grid eg, hg;
+    std::vector<Carrier*> electrons, holes;
+    std::vector<int> defects;
+    std::vector<int> traps;
+    std::vector<double> trap_pots;
+    int max_e = 0, max_h = 0, max_d = 0, max_t = 0;
+    Table3 R1, R2, iR, eR, sI, coupling;
+    double rate[10] = {0};
+    uint64_t attempts[10] = {0}, successes[10] = {0};
+    int slot_of[10] = {0};          // slot of each flux agent on its grid
+    std::vector<int> face_l, face_r; // neighborsFace(Left/Right), same on both grids
+    int64_t current_step = 0;
+    // last decisions (index = list position at decide time)
+    std::vector<double> dec_u[2], dec_m[2], dec_pd[2];
+
+    ~orc_world() {
+        for (auto* c : electrons) delete c;
+        for (auto* c : holes) delete c;
+    }
+
+    const orc_params& p() const { return P.p; }
+    Grid& grid_of(int q) { return q < 0 ? eg : hg; }
+    Grid& other_grid(int q) { return q < 0 ? hg : eg; }
+    int flux_of_slot(Grid& g, int slot) const { return g.special_flux[slot - g.V]; }
+
+    /* ---------------- pair sums: potential.cpp:351-753 ---------------- */
+    template <bool kImage, bool kGauss>
+    double pair_sum_list(const std::vector<Carrier*>& charges, const Grid& g, int site_i) const {
+        const int cutoff = p().electrostatic_cutoff;
+        double potential = 0.0;
+        for (size_t i = 0; i < charges.size(); ++i) {
+            const Carrier& c = *charges[i];
+            int site_j = c.site;
+            int dx = kImage ? g.dxi_image(site_i, site_j) : g.dxi(site_i, site_j);
+            int dy = g.dyi(site_i, site_j);
+            int dz = g.dzi(site_i, site_j);
+            if ((dx < cutoff) && (dy < cutoff) && (dz < cutoff)) {
+                if (R1(dx, dy, dz) < cutoff) {
+                    double t = kGauss ? iR(dx, dy, dz) * c.q * eR(dx, dy, dz) : iR(dx, dy, dz) * c.q;
+                    if (kImage) potential -= t; else potential += t;
+                }
+            }
+        }
+        return potential * P.prefactor;
+    }
+    template <bool kImage, bool kGauss>
+    double pair_sum_defects(int site_i) const {
+        const int cutoff = p().electrostatic_cutoff;
+        const int charge = p().defects_charge;
+        double potential = 0.0;
+        if (charge == 0) return potential;
+        for (size_t i = 0; i < defects.size(); ++i) {
+            int site_j = defects[i];
+            int dx = kImage ? eg.dxi_image(site_i, site_j) : eg.dxi(site_i, site_j);
+            int dy = eg.dyi(site_i, site_j);
+            int dz = eg.dzi(site_i, site_j);
+            if ((dx < cutoff) && (dz < cutoff) && (dy < cutoff)) {
+                if (R1(dx, dy, dz) < cutoff) {
+                    double t = kGauss ? iR(dx, dy, dz) * charge * eR(dx, dy, dz) : iR(dx, dy, dz) * charge;
+                    if (kImage) potential -= t; else potential += t;
+                }
+            }
+        }
+        return potential * P.prefactor;
+    }
+    double coulombE(int s) const { return pair_sum_list<false, false>(electrons, eg, s); }
+    double coulombH(int s) const { return pair_sum_list<false, false>(holes, hg, s); }
+    double coulombD(int s) const { return pair_sum_defects<false, false>(s); }
+    double gaussE(int s) const { return pair_sum_list<false, true>(electrons, eg, s); }
+    double gaussH(int s) const { return pair_sum_list<false, true>(holes, hg, s); }
+    double gaussD(int s) const { return pair_sum_defects<false, true>(s); }
+    double coulombImageE(int s) const { return pair_sum_list<true, false>(electrons, eg, s); }
+    double coulombImageH(int s) const { return pair_sum_list<true, false>(holes, hg, s); }
+    double coulombImageD(int s) const { return pair_sum_defects<true, false>(s); }
+
+    /* potential.cpp:246-324 */
+    void precalculate_arrays() {
+        const int mx = p().electrostatic_cutoff + 1;
+        const int mh = p().hopping_range + 1;
+        R1.resize(mx); R2.resize(mx); iR.resize(mx); eR.resize(mx); sI.resize(mh);
+        for (int dx = 0; dx < mx; ++dx)
+            for (int dy = 0; dy < mx; ++dy)
+                for (int dz = 0; dz < mx; ++dz) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    double r1 = 0.0, ir = 0.0;
+                    if (r2 > 0) { r1 = std::sqrt(r2); ir = 1.0 / r1; }
+                    R2(dx, dy, dz) = r2; R1(dx, dy, dz) = r1; iR(dx, dy, dz) = ir; eR(dx, dy, dz) = 1.0;
+                }
+        if (p().coulomb_gaussian_sigma > 0.0) {
+            double factor = 1.0 / (std::sqrt(2.0) * p().coulomb_gaussian_sigma);
+            for (int dx = 0; dx < mx; ++dx)
+                for (int dy = 0; dy < mx; ++dy)
+                    for (int dz = 0; dz < mx; ++dz) eR(dx, dy, dz) = std::erf(factor * R1(dx, dy, dz));
+        }
+        for (int dx = 0; dx < mh; ++dx)
+            for (int dy = 0; dy < mh; ++dy)
+                for (int dz = 0; dz < mh; ++dz) sI(dx, dy, dz) = P.prefactor * iR(dx, dy, dz) * eR(dx, dy, dz);
+    }
+    /* potential.cpp:326-349 */
+    void update_coupling() {
+        const int mh = p().hopping_range + 1;
+        coupling.resize(mh);
+        double c1 = 1.0 / 3.0, c2 = 1.0 / 27.0;
+        for (int dx = 0; dx < mh; ++dx)
+            for (int dy = 0; dy < mh; ++dy)
+                for (int dz = 0; dz < mh; ++dz) {
+                    double r = std::sqrt(double(dx * dx + dy * dy + dz * dz));
+                    coupling(dx, dy, dz) = c1 * std::pow(3.0, r - 1) * std::pow(c2, r - 1);
+                }
+        coupling(0, 0, 0) = 0;
+    }
+
+    /* ---------------- agents ---------------- */
+    void register_carrier(Carrier* c) {          // cubicgrid.cpp:611-625
+        Grid& g = grid_of(c->q);
+        if (g.at[c->site] != nullptr || g.type[c->site] != Empty) fail("can not register agent");
+        g.at[c->site] = c;
+        g.type[c->site] = c->q < 0 ? Electron : Hole;
+        c->nbrs = g.neighbors(c->site, p().hopping_range);
+    }
+    void unregister_carrier(Carrier* c) {        // cubicgrid.cpp:627-636
+        Grid& g = grid_of(c->q);
+        if (g.at[c->site] != c) fail("unregister: pointers do not match");
+        g.type[c->site] = Empty;
+        g.at[c->site] = nullptr;
+    }
+    Carrier* new_carrier(int q, int site) {
+        Carrier* c = new Carrier;
+        c->q = q; c->site = site; c->fsite = site;
+        register_carrier(c);
+        return c;
+    }
+
+    /* sourceagent.cpp:204-247 */
+    bool valid_electron(int site, double prob) const {
+        return !(int(electrons.size()) >= max_e || prob <= 0 || site < 0 || site >= eg.V ||
+                 eg.type[site] != Empty || eg.at[site] != nullptr ||
+                 std::find(defects.begin(), defects.end(), site) != defects.end());
+    }
+    bool valid_hole(int site, double prob) const {
+        return !(int(holes.size()) >= max_h || prob <= 0 || site < 0 || site >= hg.V ||
+                 hg.type[site] != Empty || hg.at[site] != nullptr ||
+                 std::find(defects.begin(), defects.end(), site) != defects.end());
+    }
+    bool valid_exciton(int site, double prob) const {
+        return !(int(electrons.size()) >= max_e || int(holes.size()) >= max_h || prob <= 0 ||
+                 site < 0 || site >= eg.V || eg.type[site] != Empty || hg.type[site] != Empty ||
+                 eg.at[site] != nullptr || hg.at[site] != nullptr ||
+                 std::find(defects.begin(), defects.end(), site) != defects.end());
+    }
+
+    /* sourceagent.cpp:125-161 (m_potential of a source is never set: 0) */
+    double source_energy_change(int q, int site) {
+        double p1 = 0.0;
+        double p2 = grid_of(q).pot[site];
+        if (p().source_coulomb) {
+            p2 += coulombH(site);
+            p2 += coulombImageH(site);
+            p2 += coulombE(site);
+            p2 += coulombImageE(site);
+            if (p().defects_charge != 0) {
+                p2 += coulombD(site);
+                p2 += coulombImageD(site);
+            }
+        }
+        return q < 0 ? p1 - p2 : p2 - p1;
+    }
+
+    /* sourceagent.cpp:92-103 for the four face sources */
+    bool try_inject_face(int flux) {
+        const int q = (flux == ESL || flux == ESR) ? -1 : +1;
+        const std::vector<int>& nb = (flux == ESL || flux == HSL) ? face_l : face_r;
+        attempts[flux] += 1;
+        int site = nb[rng.integer(0, int(nb.size()) - 1)];
+        bool valid = q < 0 ? valid_electron(site, rate[flux]) : valid_hole(site, rate[flux]);
+        if (!valid) return false;
+        bool go;
+        if (p().source_metropolis)   // :168-176
+            go = rng.metropolisWithCoupling(source_energy_change(q, site), P.inverse_kt, rate[flux]);
+        else
+            go = rng.chooseYes(rate[flux]);
+        if (!go) return false;
+        Carrier* c = new_carrier(q, site);
+        (q < 0 ? electrons : holes).push_back(c);
+        successes[flux] += 1;
+        return true;
+    }
+    /* sourceagent.cpp:92-103,120-123,178-181,195-202,234-250 */
+    bool try_inject_exciton() {
+        attempts[XS] += 1;
+        int site = rng.integer(0, eg.V - 1);
+        if (!valid_exciton(site, rate[XS])) return false;
+        if (!rng.chooseYes(rate[XS])) return false;
+        electrons.push_back(new_carrier(-1, site));
+        holes.push_back(new_carrier(+1, site));
+        successes[XS] += 1;
+        return true;
+    }
+
+    /* chargeagent.cpp:463-481 */
+    double binding(int q, int site) const {
+        if (q > 0) return (eg.type[site] == Electron) ? sI(1, 0, 0) - p().exciton_binding : 0.0;
+        return (hg.type[site] == Hole) ? sI(1, 0, 0) + p().exciton_binding : 0.0;
+    }
+
+    /* chargeagent.cpp:215-271 */
+    void coulomb_cpu(Carrier* c) const {
+        double p1 = 0, p2 = 0;
+        double self = sI(1, 0, 0) * c->q;
+        if (p().coulomb_gaussian_sigma > 0) {
+            p1 += gaussE(c->site); p2 += gaussE(c->fsite);
+            p1 += gaussH(c->site); p2 += gaussH(c->fsite);
+            if (p().defects_charge != 0) { p1 += gaussD(c->site); p2 += gaussD(c->fsite); }
+        } else {
+            p1 += coulombE(c->site); p2 += coulombE(c->fsite);
+            p1 += coulombH(c->site); p2 += coulombH(c->fsite);
+            if (p().defects_charge != 0) { p1 += coulombD(c->site); p2 += coulombD(c->fsite); }
+        }
+        p2 -= self;
+        p2 += binding(c->q, c->fsite);
+        p1 += binding(c->q, c->site);
+        c->de = c->q * (p2 - p1);
+    }
+
+    /* chargeagent.cpp:90-161 */
+    void decide_future(Carrier* c, int which, size_t index) {
+        c->lifetime += 1;
+        Grid& g = grid_of(c->q);
+        double u = -1.0, m = 0.0, pdv = 0.0;
+        switch (g.type[c->fsite]) {
+            case Empty: {
+                double pd = g.pot[c->fsite] - g.pot[c->site];
+                pd *= c->q;
+                pd += c->de;
+                int dx = g.dxi(c->site, c->fsite), dy = g.dyi(c->site, c->fsite), dz = g.dzi(c->site, c->fsite);
+                double cpl = coupling(dx, dy, dz);
+                pdv = pd;
+                if (rng.metropolisWithCoupling(pd, P.inverse_kt, cpl, &u, &m)) {
+                    c->pathlength += 1;
+                } else {
+                    c->fsite = c->site;
+                }
+                break;
+            }
+            case Drain: {
+                int flux = flux_of_slot(g, c->fsite);
+                attempts[flux] += 1;   // drainagent.cpp:83-92
+                if (rng.chooseYes(rate[flux], &u, &m)) {
+                    successes[flux] += 1;
+                    c->pathlength += 1;
+                    break;
+                }
+                c->fsite = c->site;
+                break;
+            }
+            default:
+                c->fsite = c->site;
+                break;
+        }
+        dec_u[which][index] = u;
+        dec_m[which][index] = m;
+        dec_pd[which][index] = pdv;
+    }
+
+    /* chargeagent.cpp:163-199 */
+    void complete_tick(Carrier* c) {
+        Grid& g = grid_of(c->q);
+        if (c->removed) { unregister_carrier(c); return; }
+        if (c->site != c->fsite) {
+            if (g.type[c->fsite] == Empty) {
+                unregister_carrier(c);
+                c->site = c->fsite;
+                register_carrier(c);
+                return;
+            }
+            if (g.type[c->fsite] == Drain) {
+                unregister_carrier(c);
+                c->removed = true;
+                return;
+            }
+            c->fsite = c->site;
+        }
+    }
+
+    /* drainagent.cpp:113-214 */
+    void recombination_try(Carrier* e) {
+        int site = e->site;
+        std::vector<int> nb;
+        if (hg.type[site] == Hole) nb.push_back(site);
+        if (p().recombination_range != 0) {
+            std::vector<int> cand = (p().hopping_range == p().recombination_range)
+                                        ? e->nbrs : eg.neighbors(site, p().recombination_range);
+            for (int s : cand) if (hg.type[s] == Hole) nb.push_back(s);
+        }
+        if (nb.empty()) return;
+        int rs = nb[rng.integer(0, int(nb.size()) - 1)];
+        Carrier* other = hg.at[rs];
+        if (!other) fail("recombination: no hole at site");
+        if (other->removed) return;
+        attempts[XD] += 1;
+        if (rng.chooseYes(rate[XD])) {
+            successes[XD] += 1;
+            e->removed = true;
+            other->removed = true;
+        }
+    }
+
+    /* simulation.cpp:253-317 */
+    void balance_charges() {
+        int tries = 0;
+        while (int(electrons.size()) - int(holes.size()) < 0 && !(int(electrons.size()) >= max_e)) {
+            if (p().voltage_right > p().voltage_left) try_inject_face(ESL);
+            else if (p().voltage_left > p().voltage_right) try_inject_face(ESR);
+            else { if (rng.random() > 0.5) try_inject_face(ESL); else try_inject_face(ESR); }
+            if (++tries > 1024) fail("can not balance electrons");
+        }
+        tries = 0;
+        while (int(holes.size()) - int(electrons.size()) < 0 && !(int(holes.size()) >= max_h)) {
+            if (p().voltage_right > p().voltage_left) try_inject_face(HSR);
+            else if (p().voltage_left > p().voltage_right) try_inject_face(HSL);
+            else { if (rng.random() > 0.5) try_inject_face(HSR); else try_inject_face(HSL); }
+            if (++tries > 1024) fail("can not balance holes");
+        }
+    }
+
+    /* simulation.cpp:46-53 */
+    void phase_propose() {
+        for (auto* c : electrons) { c->fsite = c->nbrs[rng.integer(0, int(c->nbrs.size()) - 1)]; c->de = 0; }
+        for (auto* c : holes) { c->fsite = c->nbrs[rng.integer(0, int(c->nbrs.size()) - 1)]; c->de = 0; }
+    }
+    /* simulation.cpp:85-88 (QtConcurrent::map over carriers -> OpenMP) */
+    void phase_coulomb(int threads) {
+        if (!p().coulomb_carriers) return;
+        const int ne = int(electrons.size()), nt = ne + int(holes.size());
+#ifdef _OPENMP
+        if (threads > 1) {
+#pragma omp parallel for schedule(dynamic, 8) num_threads(threads)
+            for (int i = 0; i < nt; ++i) coulomb_cpu(i < ne ? electrons[i] : holes[i - ne]);
+            return;
+        }
+#endif
+        (void)threads;
+        for (int i = 0; i < nt; ++i) coulomb_cpu(i < ne ? electrons[i] : holes[i - ne]);
+    }
+    /* simulation.cpp:91-110 */
+    void phase_finish() {
+        dec_u[0].assign(electrons.size(), -1); dec_m[0].assign(electrons.size(), 0); dec_pd[0].assign(electrons.size(), 0);
+        dec_u[1].assign(holes.size(), -1); dec_m[1].assign(holes.size(), 0); dec_pd[1].assign(holes.size(), 0);
+        for (size_t i = 0; i < electrons.size(); ++i) decide_future(electrons[i], 0, i);
+        for (size_t i = 0; i < holes.size(); ++i) decide_future(holes[i], 1, i);
+        // :219-231
+        if (p().simulation_type == 1 && p().recombination_rate > 0)
+            for (auto* e : electrons) recombination_try(e);
+        // :355-414
+        tick_list(electrons);
+        tick_list(holes);
+        // :233-251
+        if (p().simulation_type == 1) {
+            try_inject_exciton();
+            if (p().balance_charges) balance_charges();
+        } else {
+            try_inject_face(ESL);
+            try_inject_face(ESR);
+            try_inject_face(HSL);
+            try_inject_face(HSR);
+        }
+        current_step += 1;
+    }
+    void tick_list(std::vector<Carrier*>& list) {
+        size_t w = 0;
+        for (size_t i = 0; i < list.size(); ++i) {
+            Carrier* c = list[i];
+            complete_tick(c);
+            if (c->removed) { delete c; continue; }
+            list[w++] = c;
+        }
+        list.resize(w);
+    }
+};
+
+namespace {
+
+/* world.cpp:659-757 */
+void place_defects(orc_world& W, const std::vector<int>& ids) {
+    int max = W.max_d;
+    int to_seed = max - int(ids.size());
+    if (to_seed < 0) to_seed = 0;
+    if (int(ids.size()) > max) fail("invalid number of defects in checkpoint");
+    if (max == 0) return;
+    for (int s : ids) {
+        if (std::find(W.defects.begin(), W.defects.end(), s) != W.defects.end()) fail("defect already exists");
+        if (W.eg.type[s] != Empty || W.hg.type[s] != Empty) fail("can not register defect");
+        W.eg.type[s] = Defect; W.hg.type[s] = Defect;
+        W.defects.push_back(s);
+    }
+    int num = int(W.defects.size());
+    if (to_seed > 0) {
+        int tries = 0, max_tries = 10 * W.eg.V;
+        for (int i = num; i < max;) {
+            ++tries;
+            int site = W.rng.integer(0, W.eg.V - 1);
+            if (std::find(W.defects.begin(), W.defects.end(), site) == W.defects.end()) {
+                if (W.eg.type[site] != Empty || W.hg.type[site] != Empty) fail("can not register defect");
+                W.eg.type[site] = Defect; W.hg.type[site] = Defect;
+                W.defects.push_back(site);
+                ++i;
+            }
+            if (tries > max_tries) fail("can not seed defects");
+        }
+    }
+}
+
+/* world.cpp:759-958 via SourceAgent::tryToSeed (sourceagent.cpp:71-90) */
+void place_carriers(orc_world& W, int q, const std::vector<int>& ids) {
+    auto& list = q < 0 ? W.electrons : W.holes;
+    int max = q < 0 ? W.max_e : W.max_h;
+    double seedc = W.p().seed_charges;
+    int to_ids = int(ids.size());
+    int max_seeded = int(max * seedc);
+    int to_seed = max_seeded - to_ids;
+    if (to_seed < 0) to_seed = 0;
+    int to_place = to_ids + to_seed;
+    if (to_place > max || to_ids > max) fail("invalid number of carriers to be placed");
+    if (to_place == 0) return;
+    auto valid = [&](int site) { return q < 0 ? W.valid_electron(site, 1.0) : W.valid_hole(site, 1.0); };
+    for (int s : ids) {
+        if (!valid(s)) fail("can not inject carrier at checkpoint site");
+        list.push_back(W.new_carrier(q, s));
+    }
+    if (to_seed > 0 && seedc != 0) {
+        int tries = 0, max_tries = 10 * W.eg.V;
+        for (int i = int(list.size()); i < max * seedc;) {
+            int site = W.rng.integer(0, W.eg.V - 1);   // randomSiteID, sourceagent.cpp:105-108
+            if (valid(site)) { list.push_back(W.new_carrier(q, site)); ++i; }
+            if (++tries > max_tries) fail("can not seed carriers");
+        }
+    }
+}
+
+/* potential.cpp:74-244 */
+void set_potential_traps(orc_world& W, const std::vector<int>& ids, const std::vector<double>& pots) {
+    int total = W.max_t, forced = int(ids.size());
+    int randomly = total - forced;
+    int seeds = int(randomly * W.p().seed_percentage);
+    int grown = randomly - seeds;
+    if (randomly <= 0) { seeds = 0; grown = 0; }
+    if (seeds == 0 && grown > 0) { seeds = grown; grown = 0; }
+    if (total < 0 || forced < 0 || randomly < 0 || seeds < 0 || grown < 0 || forced > total ||
+        randomly > total || seeds > total || grown > total)
+        fail("invalid trap parameters");
+    if (forced > 0 && !pots.empty() && pots.size() != ids.size()) fail("trap potential list size");
+    std::vector<int> traps;
+    std::vector<double> potentials;
+    for (int i = 0; i < forced; ++i) {
+        traps.push_back(ids[i]);
+        potentials.push_back(pots.empty() ? W.p().trap_potential : pots[i]);
+    }
+    std::vector<int> rids;
+    std::vector<double> rpots;
+    if (randomly > 0) {
+        int progress = 0;
+        while (progress < seeds) {
+            int s = W.rng.integer(0, W.eg.V - 1);
+            if (std::find(rids.begin(), rids.end(), s) == rids.end() &&
+                std::find(traps.begin(), traps.end(), s) == traps.end()) {
+                rids.push_back(s); rpots.push_back(W.p().trap_potential); ++progress;
+            }
+        }
+        progress = 0;
+        while (progress < grown) {
+            int si = W.rng.integer(0, int(rids.size()) - 1);
+            std::vector<int> nb = W.eg.neighbors(rids[si], 1);
+            int ni = W.rng.integer(0, int(nb.size()) - 1);
+            int ns = nb[ni];
+            if (W.eg.type[ns] != Source && W.eg.type[ns] != Drain && W.hg.type[ns] != Source &&
+                W.hg.type[ns] != Drain && std::find(rids.begin(), rids.end(), ns) == rids.end() &&
+                std::find(traps.begin(), traps.end(), ns) == traps.end()) {
+                rids.push_back(ns); rpots.push_back(W.p().trap_potential); ++progress;
+            }
+        }
+        if (std::abs(W.p().gaussian_stdev) > 0)
+            fail("gaussian.stdev > 0 draws from boost::normal_distribution (not restated)");
+        for (size_t i = 0; i < rids.size(); ++i) { traps.push_back(rids[i]); potentials.push_back(rpots[i]); }
+    }
+    W.traps = traps;
+    W.trap_pots = potentials;
+    for (size_t i = 0; i < traps.size(); ++i) {
+        W.eg.pot[traps[i]] += potentials[i];
+        W.hg.pot[traps[i]] += potentials[i];
+    }
+}
+
+void set_rate_smartly(double& dst, double r, double dflt) {   // fluxagent.cpp:156-169
+    if (r < 0) {
+        if (dflt >= 0) { dst = dflt; return; }
+        fail("negative rate");
+    }
+    dst = r;
+}
+
+void check_params(const orc_params& p) {   // parameters.h:393-646 (the checks this path reads)
+    if (p.simulation_type != 0 && p.simulation_type != 1) fail("simulation.type");
+    if (p.grid_x < 1 || p.grid_y < 1 || p.grid_z < 1) fail("grid size");
+    if (p.hopping_range < 1 || p.hopping_range > 2) fail("hopping.range");
+    if (p.electrostatic_cutoff < 1) fail("electrostatic.cutoff");
+    if (!p.source_metropolis && p.source_coulomb) fail("source.metropolis = false, yet source.coulomb = true");
+    if (p.defects_charge != 0 && !p.coulomb_carriers) fail("defects.charge != 0 && coulomb.carriers = false");
+    if (p.recombination_rate > 0 && p.simulation_type != 1) fail("recombination.rate > 0 needs solarcell");
+    if (p.balance_charges && p.simulation_type != 1) fail("balance.charges needs solarcell");
+    if (p.random_seed == 0) fail("random.seed must be > 0 (0 seeds from the clock)");
+    if (p.coulomb_gaussian_sigma > 0 && (p.grid_x == 1 || p.grid_y == 1 || p.grid_z == 1))
+        fail("coulomb.gaussian.sigma > 0, yet the simulation is not 3D");
+    auto pct = [](double v) { return v < 0.0 || v > 1.0; };
+    if (pct(p.electron_percentage) || pct(p.hole_percentage) || pct(p.defect_percentage) ||
+        pct(p.trap_percentage) || pct(p.seed_percentage) || pct(p.seed_charges))
+        fail("percentage out of [0, 1]");
+}
+
+template <class F>
+int orc_guard(F&& f) {
+    try { f(); return 0; }
+    catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+}  // namespace
+
+/* ========================================================================= */
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+
+int orc_mt_outputs(uint32_t seed, uint64_t skip, int32_t n, uint32_t* out) {
+    return orc_guard([&] {
+        MT19937 mt; mt.seed(seed);
+        for (uint64_t i = 0; i < skip; ++i) mt();
+        for (int i = 0; i < n; ++i) out[i] = mt();
+    });
+}
+
+int orc_uniform_int(uint32_t seed, int32_t lo, int32_t hi, int32_t n, int32_t* out) {
+    return orc_guard([&] {
+        MT19937 mt; mt.seed(seed);
+        for (int i = 0; i < n; ++i) out[i] = uniform_int(mt, lo, hi);
+    });
+}
+
+int orc_uniform_int_words(const uint32_t* words, int32_t nwords, int32_t lo, int32_t hi,
+                          int32_t n, int32_t* out, int32_t* used) {
+    return orc_guard([&] {
+        struct Src { const uint32_t* w; int n; int k = 0;
+            uint32_t operator()() { if (k >= n) fail("word stream exhausted"); return w[k++]; } };
+        Src s{words, nwords};
+        for (int i = 0; i < n; ++i) out[i] = uniform_int(s, lo, hi);
+        *used = s.k;
+    });
+}
+
+int orc_pair_potential(int32_t gx, int32_t gy, int32_t gz, int32_t cutoff, double prefactor,
+                       double sigma, const int32_t* charge_site, const int32_t* charge_q,
+                       int32_t n_charges, const int32_t* target_site, int32_t n_targets,
+                       int32_t flags, double* out) {
+    return orc_guard([&] {
+        // Same arithmetic as Potential::coulomb*/gauss*: per-axis test, R1 < cutoff,
+        // iR[dx][dy][dz] * q (* eR), accumulated in list order, times the prefactor.
+        Grid g; g.X = gx; g.Y = gy; g.Z = gz; g.XY = gx * gy; g.V = gx * gy * gz;
+        const bool gauss = flags & ORC_GAUSS, image = flags & ORC_IMAGE;
+        const double factor = sigma > 0 ? 1.0 / (std::sqrt(2.0) * sigma) : 0.0;
+        for (int t = 0; t < n_targets; ++t) {
+            int si = target_site[t];
+            double pot = 0.0;
+            for (int j = 0; j < n_charges; ++j) {
+                int sj = charge_site[j];
+                int dx = image ? g.dxi_image(si, sj) : g.dxi(si, sj);
+                int dy = g.dyi(si, sj), dz = g.dzi(si, sj);
+                if (dx < cutoff && dy < cutoff && dz < cutoff) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    double r1 = r2 > 0 ? std::sqrt(r2) : 0.0;
+                    if (r1 < cutoff) {
+                        double ir = r2 > 0 ? 1.0 / r1 : 0.0;
+                        double term = ir * charge_q[j];
+                        if (gauss) term = ir * charge_q[j] * std::erf(factor * r1);
+                        if (image) pot -= term; else pot += term;
+                    }
+                }
+            }
+            out[t] = pot * prefactor;
+        }
+    });
+}
+
+int orc_potential_map(int32_t gx, int32_t gy, int32_t gz, int32_t cutoff, double prefactor,
+                      double sigma, const int32_t* charge_site, const int32_t* charge_q,
+                      int32_t n_charges, int32_t flags, int32_t threads, double* out) {
+    return orc_guard([&] {
+        // kernel.cl coulomb1/gauss1 closed form in FP64: 0 < r^2 < c^2.
+        const int64_t V = int64_t(gx) * gy * gz;
+        const int64_t c2 = cutoff > 0 ? int64_t(cutoff) * cutoff : INT64_MAX;
+        const bool gauss = flags & ORC_GAUSS;
+        const double factor = sigma > 0 ? 1.0 / (std::sqrt(2.0) * sigma) : 0.0;
+        std::vector<int> cx(n_charges), cy(n_charges), cz(n_charges);
+        const int XY = gx * gy;
+        for (int j = 0; j < n_charges; ++j) {
+            int s = charge_site[j];
+            cz[j] = s / XY; cy[j] = s / gx - cz[j] * gy; cx[j] = s % gx;
+        }
+        (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+#endif
+        for (int64_t s = 0; s < V; ++s) {
+            int z = int(s / XY), y = int(s / gx - int64_t(z) * gy), x = int(s % gx);
+            double v = 0.0;
+            for (int j = 0; j < n_charges; ++j) {
+                int64_t dx = x - cx[j], dy = y - cy[j], dz = z - cz[j];
+                int64_t r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > 0 && r2 < c2) {
+                    double r = std::sqrt(double(r2));
+                    v += gauss ? charge_q[j] * std::erf(factor * r) / r : charge_q[j] / r;
+                }
+            }
+            out[s] = prefactor * v;
+        }
+    });
+}
+
+orc_world* orc_world_create(const orc_params* pp, const int32_t* electrons, int32_t n_e,
+                            const int32_t* holes, int32_t n_h, const int32_t* defects, int32_t n_d,
+                            const int32_t* traps, int32_t n_t, const double* trap_potentials,
+                            int32_t n_tp) {
+    orc_world* W = new orc_world;
+    try {
+        W->P.p = *pp;
+        const orc_params& p = W->P.p;
+        check_params(p);
+        // parameters.h:386-387
+        W->P.prefactor = p.elementary_charge /
+                         (4.0 * M_PI * p.dielectric_constant * p.permittivity_space * p.grid_factor);
+        W->P.inverse_kt = p.elementary_charge / (p.boltzmann_constant * p.temperature_kelvin);
+        // world.cpp:536-563: Random(0) then seed(random.seed)
+        W->rng.mt.seed(p.random_seed);
+        // world.cpp:589-604
+        W->eg.init(p.grid_x, p.grid_y, p.grid_z);
+        W->hg.init(p.grid_x, p.grid_y, p.grid_z);
+        const double V = double(W->eg.V);
+        W->max_h = int(p.hole_percentage * V);
+        W->max_e = int(p.electron_percentage * V);
+        W->max_d = int(p.defect_percentage * V);
+        W->max_t = int(p.trap_percentage * V);
+        // world.cpp:961-1086 createSources
+        W->slot_of[ESL] = W->eg.register_special(Source, Left, ESL);
+        W->slot_of[ESR] = W->eg.register_special(Source, Right, ESR);
+        W->slot_of[HSL] = W->hg.register_special(Source, Left, HSL);
+        W->slot_of[HSR] = W->hg.register_special(Source, Right, HSR);
+        W->slot_of[XS] = W->eg.register_special(Source, NoFace, XS);
+        double* r = W->rate;
+        if (p.simulation_type == 0) {
+            r[XS] = 0.0;
+            set_rate_smartly(r[ESL], p.e_source_l_rate, p.source_rate);
+            set_rate_smartly(r[ESR], p.e_source_r_rate, 0.0);
+            set_rate_smartly(r[HSR], p.h_source_r_rate, 0.0);
+            set_rate_smartly(r[HSL], p.h_source_l_rate, 0.0);
+        } else {
+            set_rate_smartly(r[XS], p.generation_rate, p.source_rate);
+            set_rate_smartly(r[ESR], p.e_source_r_rate, 0.0);
+            set_rate_smartly(r[ESL], p.e_source_l_rate, 0.0);
+            set_rate_smartly(r[HSR], p.h_source_r_rate, 0.0);
+            set_rate_smartly(r[HSL], p.h_source_l_rate, 0.0);
+            if (p.source_scale_area > 0 && p.source_scale_area != W->eg.XY)
+                r[XS] = r[XS] * (W->eg.XY / double(p.source_scale_area));
+        }
+        if (r[ESR] == 0 && r[ESL] == 0 && r[HSR] == 0 && r[HSL] == 0 && r[XS] == 0)
+            fail("all sources have zero rate");
+        if (W->max_e == 0 && (r[ESR] > 0 || r[ESL] > 0 || r[XS] > 0)) fail("electron source rate > 0, yet max electrons == 0");
+        if (W->max_h == 0 && (r[HSR] > 0 || r[HSL] > 0 || r[XS] > 0)) fail("hole source rate > 0, yet max holes == 0");
+        // world.cpp:1088-1187 createDrains
+        W->slot_of[EDL] = W->eg.register_special(Drain, Left, EDL);
+        W->slot_of[EDR] = W->eg.register_special(Drain, Right, EDR);
+        W->slot_of[HDL] = W->hg.register_special(Drain, Left, HDL);
+        W->slot_of[HDR] = W->hg.register_special(Drain, Right, HDR);
+        W->slot_of[XD] = W->eg.register_special(Drain, NoFace, XD);
+        if (p.simulation_type == 0) {
+            r[XD] = 0.0;
+            set_rate_smartly(r[EDR], p.e_drain_r_rate, p.drain_rate);
+            set_rate_smartly(r[EDL], p.e_drain_l_rate, 0.0);
+            set_rate_smartly(r[HDR], p.h_drain_r_rate, 0.0);
+            set_rate_smartly(r[HDL], p.h_drain_l_rate, 0.0);
+        } else {
+            r[XD] = p.recombination_rate;
+            set_rate_smartly(r[EDR], p.e_drain_r_rate, p.drain_rate);
+            set_rate_smartly(r[EDL], p.e_drain_l_rate, p.drain_rate);
+            set_rate_smartly(r[HDR], p.h_drain_r_rate, p.drain_rate);
+            set_rate_smartly(r[HDL], p.h_drain_l_rate, p.drain_rate);
+        }
+        if (r[EDR] == 0 && r[EDL] == 0 && r[HDR] == 0 && r[HDL] == 0 && r[XD] == 0)
+            fail("all drains have zero rate");
+        if (W->max_e > 0 && r[EDR] <= 0 && r[EDL] <= 0 && r[XD] <= 0) fail("all electron drain rates are zero");
+        if (W->max_h > 0 && r[HDR] <= 0 && r[HDL] <= 0 && r[XD] <= 0) fail("all hole drain rates are zero");
+        W->face_l = W->eg.neighbors_face(Left);
+        W->face_r = W->eg.neighbors_face(Right);
+        // world.cpp:625-631
+        place_defects(*W, std::vector<int>(defects, defects + n_d));
+        place_carriers(*W, -1, std::vector<int>(electrons, electrons + n_e));
+        place_carriers(*W, +1, std::vector<int>(holes, holes + n_h));
+        // world.cpp:634-643 ; potential.cpp:17-72
+        const int X = p.grid_x, Y = p.grid_y, Z = p.grid_z;
+        const double m = (p.voltage_right - p.voltage_left) / double(X), b = p.voltage_left;
+        for (int i = 0; i < X; ++i)
+            for (int j = 0; j < Y; ++j)
+                for (int k = 0; k < Z; ++k) {
+                    int s = W->eg.idx(i, j, k);
+                    double v = m * (i + 0.5) + b;
+                    W->eg.pot[s] += v; W->hg.pot[s] += v;
+                }
+        if (!(Z == 1 || p.slope_z == 0))
+            for (int i = 0; i < X; ++i)
+                for (int j = 0; j < Y; ++j)
+                    for (int k = 0; k < Z; ++k) {
+                        int s = W->eg.idx(i, j, k);
+                        double v = p.slope_z * (k + 0.5);
+                        W->eg.pot[s] += v; W->hg.pot[s] += v;
+                    }
+        set_potential_traps(*W, std::vector<int>(traps, traps + n_t),
+                            std::vector<double>(trap_potentials, trap_potentials + n_tp));
+        // world.cpp:646-649
+        W->precalculate_arrays();
+        W->update_coupling();
+        return W;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        delete W;
+        return nullptr;
+    }
+}
+
+void orc_world_destroy(orc_world* w) { delete w; }
+
+int orc_world_step(orc_world* w, int32_t n, int32_t threads) {
+    return orc_guard([&] {
+        for (int i = 0; i < n; ++i) {
+            w->phase_propose();
+            w->phase_coulomb(threads);
+            w->phase_finish();
+        }
+    });
+}
+int orc_world_phase_propose(orc_world* w) { return orc_guard([&] { w->phase_propose(); }); }
+int orc_world_phase_coulomb(orc_world* w, int32_t threads) { return orc_guard([&] { w->phase_coulomb(threads); }); }
+int orc_world_phase_finish(orc_world* w) { return orc_guard([&] { w->phase_finish(); }); }
+
+int orc_world_info(orc_world* w, int32_t* info) {
+    return orc_guard([&] {
+        info[0] = int(w->electrons.size()); info[1] = int(w->holes.size());
+        info[2] = int(w->defects.size()); info[3] = w->max_e; info[4] = w->max_h;
+        info[5] = w->eg.V; info[6] = w->eg.X; info[7] = w->eg.Y; info[8] = w->eg.Z;
+        info[9] = w->eg.special_count; info[10] = w->hg.special_count;
+        info[11] = int(w->traps.size());
+    });
+}
+int64_t orc_world_current_step(orc_world* w) { return w->current_step; }
+
+int orc_world_get_carriers(orc_world* w, int32_t which, int32_t* site, int32_t* fsite,
+                           int32_t* lifetime, int32_t* pathlength, double* de) {
+    return orc_guard([&] {
+        auto& l = which == 0 ? w->electrons : w->holes;
+        for (size_t i = 0; i < l.size(); ++i) {
+            if (site) site[i] = l[i]->site;
+            if (fsite) fsite[i] = l[i]->fsite;
+            if (lifetime) lifetime[i] = l[i]->lifetime;
+            if (pathlength) pathlength[i] = l[i]->pathlength;
+            if (de) de[i] = l[i]->de;
+        }
+    });
+}
+
+int orc_world_set_carriers(orc_world* w, int32_t which, int32_t n, const int32_t* site,
+                           const int32_t* lifetime, const int32_t* pathlength) {
+    return orc_guard([&] {
+        auto& l = which == 0 ? w->electrons : w->holes;
+        for (auto* c : l) { w->unregister_carrier(c); delete c; }
+        l.clear();
+        for (int i = 0; i < n; ++i) {
+            Carrier* c = w->new_carrier(which == 0 ? -1 : +1, site[i]);
+            c->lifetime = lifetime ? lifetime[i] : 0;
+            c->pathlength = pathlength ? pathlength[i] : 0;
+            l.push_back(c);
+        }
+    });
+}
+
+int orc_world_get_defects(orc_world* w, int32_t* site) {
+    return orc_guard([&] { for (size_t i = 0; i < w->defects.size(); ++i) site[i] = w->defects[i]; });
+}
+int orc_world_get_potential(orc_world* w, double* v) {
+    return orc_guard([&] { for (int i = 0; i < w->eg.V; ++i) v[i] = w->eg.pot[i]; });
+}
+int orc_world_get_types(orc_world* w, uint8_t* et, uint8_t* ht) {
+    return orc_guard([&] {
+        std::memcpy(et, w->eg.type.data(), w->eg.type.size());
+        std::memcpy(ht, w->hg.type.data(), w->hg.type.size());
+    });
+}
+int orc_world_get_rng(orc_world* w, uint32_t* window, uint64_t* consumed) {
+    return orc_guard([&] { w->rng.mt.get_window(window); *consumed = w->rng.mt.consumed; });
+}
+int orc_world_set_rng(orc_world* w, const uint32_t* window, uint64_t consumed) {
+    return orc_guard([&] { w->rng.mt.set_window(window, consumed); });
+}
+int orc_world_get_flux(orc_world* w, uint64_t* a, uint64_t* s, double* r) {
+    return orc_guard([&] {
+        for (int i = 0; i < 10; ++i) {
+            if (a) a[i] = w->attempts[i];
+            if (s) s[i] = w->successes[i];
+            if (r) r[i] = w->rate[i];
+        }
+    });
+}
+int orc_world_set_flux(orc_world* w, const uint64_t* a, const uint64_t* s) {
+    return orc_guard([&] { for (int i = 0; i < 10; ++i) { w->attempts[i] = a[i]; w->successes[i] = s[i]; } });
+}
+int orc_world_constants(orc_world* w, double* out) {
+    return orc_guard([&] {
+        out[0] = w->P.prefactor; out[1] = w->P.inverse_kt;
+        out[2] = w->sI(1, 0, 0); out[3] = w->coupling(1, 0, 0);
+    });
+}
+int orc_world_coupling(orc_world* w, double* out) {
+    return orc_guard([&] { std::memcpy(out, w->coupling.v.data(), w->coupling.v.size() * sizeof(double)); });
+}
+int orc_neighbors(int32_t gx, int32_t gy, int32_t gz, int32_t h, int32_t site, int32_t drain_l,
+                  int32_t drain_r, int32_t* out, int32_t* count) {
+    return orc_guard([&] {
+        Grid g;
+        g.init(gx, gy, gz);
+        if (drain_l >= 0) g.face_agents[Left].push_back({drain_l, Drain});
+        if (drain_r >= 0) g.face_agents[Right].push_back({drain_r, Drain});
+        std::vector<int> n = g.neighbors(site, h);
+        for (size_t i = 0; i < n.size(); ++i) out[i] = n[i];
+        *count = int(n.size());
+    });
+}
+
+int orc_world_last_decisions(orc_world* w, int32_t which, double* u, double* margin, double* pd) {
+    return orc_guard([&] {
+        int k = which == 0 ? 0 : 1;
+        for (size_t i = 0; i < w->dec_u[k].size(); ++i) {
+            if (u) u[i] = w->dec_u[k][i];
+            if (margin) margin[i] = w->dec_m[k][i];
+            if (pd) pd[i] = w->dec_pd[k][i];
+        }
+    });
+}
+
+}  // extern "C"
