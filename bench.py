@@ -131,9 +131,25 @@ class HipTimer:
         return float(ms.value)
 
 
+class DeviceArray:
+    """Device buffer from the HIP runtime the library itself uses (a second HIP
+    runtime in the process, e.g. torch's bundled one, would not see the GPU)."""
+
+    def __init__(self, host: np.ndarray):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.ptr = C.c_void_p()
+        self.nbytes = host.nbytes
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.nbytes, 4))) == 0
+        assert self.hip.hipMemcpy(self.ptr, host.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), 1) == 0
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
 def bench_map(reps=3):
     """Full-lattice map, 256x256x64, 50,000 charges (+/-1 equal mix), cutoff 50 and none."""
-    import torch
     import langmuir_amd as L
     dims = (256, 256, 64)
     V = 256 * 256 * 64
@@ -143,22 +159,22 @@ def bench_map(reps=3):
     cq = np.where(np.arange(n) < n // 2, 1, -1).astype(np.int32)
     out = {}
     ctx = L.Context(dims, cutoff=50)
-    d_cs = torch.from_numpy(cs).cuda()
-    d_cq = torch.from_numpy(cq).cuda()
-    d_out = torch.empty(V, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
+    d_cs, d_cq = DeviceArray(cs), DeviceArray(cq)
+    d_out = DeviceArray(np.zeros(V, np.float32))
     timer = HipTimer(ctx.stream)
-    for name, flags, cut in (("cutoff50", 0, 50), ("nocutoff", L.LMG_NO_CUTOFF, None)):
-        ctx.potential_map_dev(d_cs.data_ptr(), d_cq.data_ptr(), n, d_out.data_ptr(), flags)
+    for name, flags in (("cutoff50", 0), ("nocutoff", L.LMG_NO_CUTOFF)):
+        ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags)
         ctx.sync()
         timer.start()
         for _ in range(reps):
-            ctx.potential_map_dev(d_cs.data_ptr(), d_cq.data_ptr(), n, d_out.data_ptr(), flags)
+            ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags)
         ms = timer.stop_ms() / reps
         pair_evals = V * n
         out[name] = dict(ms_per_map=round(ms, 4), maps_per_s=round(1e3 / ms, 3),
                          pair_evals_per_s=pair_evals / (ms * 1e-3))
         log(f"map {name}: {ms:.3f} ms/map, {pair_evals / (ms * 1e-3):.3e} pair-evals/s")
+    for a in (d_cs, d_cq, d_out):
+        a.free()
     ctx.close()
     return out
 

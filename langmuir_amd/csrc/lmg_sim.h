@@ -1,0 +1,836 @@
+// lmg_sim.h -- device side of one KMC sweep (Simulation::performIterations,
+// src/langmuirCore/simulation.cpp:29-111) as a short chain of kernels on one
+// stream, every carrier-parallel phase spread over the whole GPU:
+//
+//   de_kernel      (grid)   dE pair sums of every carrier's proposed hop, the
+//                           decision-draw flags, and (one extra workgroup) the
+//                           MT19937 refill for later steps
+//   decide_kernel  (grid)   draw offsets (block prefix + in-block scan), dE
+//                           assembly, Metropolis / drain decisions, claims
+//   recomb_kernel  (1 WG)   RecombinationAgent sweep (solar cell, rate > 0)
+//   tick_kernel    (grid)   completeTick conflict rule and order-preserving
+//                           compaction of the lists
+//   inject_kernel  (1 WG)   injection (serial in the reference) and the next
+//                           step's proposals
+//
+// Offsets into the random stream are computed so that every draw uses exactly
+// the word the reference's serial loop would use.
+#pragma once
+
+#include "lmg_device.h"
+
+namespace lmg {
+
+constexpr int kSimBlock = 256;   // carriers per workgroup in the grid phases
+constexpr int kOneBlock = 1024;  // single-workgroup phases
+
+// device error codes (lmg_sim_device_error)
+constexpr int kErrRing = 1, kErrCapacity = 2, kErrSite = 3, kErrBalance = 4;
+
+struct SimCtl {
+    int n_e, n_h;                 // list sizes of the pending step
+    int cur;                      // buffer that holds the lists
+    int error;
+    int n_e_next, n_h_next;       // sizes after compaction (tick_kernel)
+    int counts[2];                // de_kernel: targets, charges
+    int first_bad;
+    int pad;
+    unsigned long long rng_pos;   // engine outputs consumed before the pending decisions
+    unsigned long long rng_step0; // ... before the pending step's proposals
+    unsigned long long gen_pos;   // raw words x[0 .. gen_pos) generated
+    unsigned long long draws_dec; // decision draws of the pending step
+    unsigned long long draws_rec; // recombination draws of the pending step
+    unsigned long long att[10], suc[10];
+    long long step;
+};
+
+struct SimDev {
+    Dims d;
+    int hop, sim_type, coulomb, src_metro, src_coul, balance, rec_range, gauss;
+    int max_e, max_h, cap_e, cap_h, n_d_eff, dq;
+    int e_drain[2], h_drain[2];
+    int uniform;                  // every site has >= 2 neighbours: one proposal draw each
+    int nblk;                     // workgroups of kSimBlock carriers at capacity
+    uint32_t c2m1;
+    float gk;
+    double vl, vr, beta, pref, self_int, binding;
+    double rate[10];
+    double coupling[27];
+    int S, ld;
+    unsigned long long lookahead;
+    int* e_site[2];
+    int* e_life[2];
+    int* e_path[2];
+    int* h_site[2];
+    int* h_life[2];
+    int* h_path[2];
+    int* e_fsite;
+    int* h_fsite;
+    uint8_t* etype;
+    uint8_t* htype;
+    double* vbg;
+    unsigned long long* claim_e;  // per site: (step tag << 32) | ~list index, max wins
+    unsigned long long* claim_h;
+    int* h_at;
+    int* code;      // per carrier: 0 stay, 1 accepted to an empty site, 2 accepted into a drain
+    int* removed;   // per carrier: recombined this step
+    int* flag;      // per carrier: decision draws (0/1)
+    int* blk_draw;  // per workgroup: decision draws
+    int* blk_keep;  // per workgroup: kept electrons, kept holes
+    float4* tgt1;
+    float4* tgt2;
+    float4* chg;    // [charged defects | electrons | holes]
+    float2* part;
+    double* de;
+    uint32_t* ring;
+    unsigned long long ring_mask;
+    SimCtl* ctl;
+};
+
+__device__ __forceinline__ void set_error(const SimDev& D, int code) { atomicCAS(&D.ctl->error, 0, code); }
+
+__device__ __forceinline__ uint32_t rng_word(const SimDev& D, unsigned long long k) {
+    return mt_temper(D.ring[(k + 624ull) & D.ring_mask]);
+}
+
+__device__ __forceinline__ float4 site_f4(const Dims& d, int s, float q) {
+    int x, y, z;
+    decode(d, s, x, y, z);
+    return make_float4(float(x), float(y), float(z), q);
+}
+
+__device__ __forceinline__ float rsqf(float r2) { return __builtin_amdgcn_rsqf(r2); }
+__device__ __forceinline__ bool cut_in(float r2, uint32_t c2m1) { return (__float_as_uint(r2) - 1u) < c2m1; }
+__device__ __forceinline__ void kahan(float& s, float& c, float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// exclusive scan of one int per thread over a workgroup of nw waves
+template <int kThreads>
+__device__ int wg_scan(int v, int* wsum, int& total) {
+    constexpr int nw = kThreads / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < nw ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (lane < nw) wsum[lane] = w;
+    }
+    __syncthreads();
+    const int excl = x - v + (wid > 0 ? wsum[wid - 1] : 0);
+    total = wsum[nw - 1];
+    __syncthreads();
+    return excl;
+}
+
+// sum of arr[stride*b + off] over b < nb, computed by wave 0 and broadcast
+__device__ long long wg_prefix(const int* arr, int nb, int stride, int off, long long* slot) {
+    if (threadIdx.x < 64) {
+        long long s = 0;
+        for (int b = threadIdx.x; b < nb; b += 64) s += arr[stride * b + off];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (threadIdx.x == 0) *slot = s;
+    }
+    __syncthreads();
+    return *slot;
+}
+
+// ---------------------------------------------------------------------------
+// MT19937 refill: x[m] = x[m-227] ^ twist(x[m-624], x[m-623]), so 227
+// consecutive words can be produced at once from the last 624.
+// ---------------------------------------------------------------------------
+__device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsigned long long target) {
+    unsigned long long gen = D.ctl->gen_pos;
+    if (gen >= target) return;
+    for (int t = threadIdx.x; t < 624; t += blockDim.x) {
+        const unsigned long long m = gen - 624ull + t;
+        tail[m & 1023ull] = D.ring[m & D.ring_mask];
+    }
+    __syncthreads();
+    while (gen < target) {
+        if (threadIdx.x < 227) {
+            const unsigned long long m = gen + threadIdx.x;
+            const uint32_t v = mt_next(tail[(m - 624ull) & 1023ull], tail[(m - 623ull) & 1023ull],
+                                       tail[(m - 227ull) & 1023ull]);
+            tail[m & 1023ull] = v;
+            D.ring[m & D.ring_mask] = v;
+        }
+        gen += 227ull;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) D.ctl->gen_pos = gen;
+}
+
+// ===========================================================================
+// de_kernel: grid = (nblk + 1, S).  Workgroup (x, s) sums charge chunk s for
+// carriers [256x, 256x + 256); chunk-0 workgroups also flag the carriers whose
+// decision draws (own-grid type Empty or Drain at the proposed site,
+// chargeagent.cpp:95-157).  Workgroup (nblk, 0) refills the random stream.
+//
+// For each charge j it accumulates q_j ( [0<r2<c2] f(r2) - [0<r1<c2] f(r1) ),
+// the pair part of p2 - p1 in ChargeAgent::coulombCPU (chargeagent.cpp:215-271).
+// For 1/r, two in-cutoff terms are differenced without cancellation:
+//   1/sqrt(r2) - 1/sqrt(r1) = (r1 - r2) (ab)^2 / (a + b),  a = r1^-1/2, b = r2^-1/2,
+// with r1 - r2 an exact integer.
+// ===========================================================================
+template <bool kGauss, bool kPairs>
+__global__ __launch_bounds__(kSimBlock) void de_kernel(SimDev D) {
+    __shared__ float4 tile[256];
+    __shared__ uint32_t tail[1024];
+    if (int(blockIdx.x) == D.nblk) {
+        if (blockIdx.y == 0) mt_refill(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
+        return;
+    }
+    const int ne = D.ctl->n_e, nt = D.ctl->counts[0], nc = D.ctl->counts[1];
+    if (int(blockIdx.x) * kSimBlock >= nt) return;   // uniform per workgroup
+    const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    if (blockIdx.y == 0) {
+        int draw = 0;
+        if (i < nt) {
+            const bool isE = i < ne;
+            const int fs = isE ? D.e_fsite[i] : D.h_fsite[i - ne];
+            const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
+            draw = (tf == kEmpty || tf == kDrain);
+            D.flag[i] = draw;
+        }
+        const int cnt = __syncthreads_count(draw);
+        if (threadIdx.x == 0) D.blk_draw[blockIdx.x] = cnt;
+    }
+    if (!kPairs) return;
+    const int S = gridDim.y;
+    const int chunk = (nc + S - 1) / S;
+    const int j0 = blockIdx.y * chunk, j1 = min(nc, j0 + chunk);
+    const float4 p1 = i < nt ? D.tgt1[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 p2 = i < nt ? D.tgt2[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
+    const float hh = hx * hx + hy * hy + hz * hz;
+    const uint32_t c2m1 = D.c2m1;
+    const float gk = D.gk;
+    float s = 0.f, c = 0.f;
+    for (int base = j0; base < j1; base += 256) {
+        const int n = min(256, j1 - base);
+        __syncthreads();
+        if (threadIdx.x < n) tile[threadIdx.x] = D.chg[base + threadIdx.x];
+        __syncthreads();
+        for (int k0 = 0; k0 < n; k0 += 32) {
+            const int k1 = min(n, k0 + 32);
+            float acc = 0.f;
+            for (int k = k0; k < k1; ++k) {
+                const float4 q = tile[k];
+                const float dx = p1.x - q.x, dy = p1.y - q.y, dz = p1.z - q.z;
+                const float r1 = dx * dx + dy * dy + dz * dz;
+                // |p2 - q|^2 = r1 + 2 h.(p1 - q) + |h|^2, exact in integers
+                const float r2 = r1 + (2.f * (hx * dx + hy * dy + hz * dz) + hh);
+                const bool in1 = cut_in(r1, c2m1), in2 = cut_in(r2, c2m1);
+                const float a = rsqf(r1), b = rsqf(r2);
+                float term;
+                if (kGauss) {
+                    const float fa = in1 ? erff(gk * (r1 * a)) * a : 0.f;
+                    const float fb = in2 ? erff(gk * (r2 * b)) * b : 0.f;
+                    term = fb - fa;
+                } else {
+                    const float ab = a * b;
+                    const float both = (r1 - r2) * (ab * ab) * __builtin_amdgcn_rcpf(a + b);
+                    term = (in1 && in2) ? both : ((in2 ? b : 0.f) - (in1 ? a : 0.f));
+                }
+                acc = fmaf(q.w, term, acc);
+            }
+            kahan(s, c, acc);
+        }
+    }
+    if (i < nt) D.part[blockIdx.y * D.ld + i] = make_float2(s, -c);
+}
+
+// ===========================================================================
+// decide_kernel: decideFuture (chargeagent.cpp:90-161) for every carrier.
+// The draw of carrier i is word rng_pos + (draws of carriers before i).
+// only_de = 1 assembles dE (chargeagent.cpp:215-296) and stops.
+// ===========================================================================
+__global__ __launch_bounds__(kSimBlock) void decide_kernel(SimDev D, int only_de) {
+    __shared__ int wsum[4];
+    __shared__ long long pre;
+    __shared__ unsigned int att[4], suc[4];
+    const SimCtl& C = *D.ctl;
+    const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
+    const int nb = (N + kSimBlock - 1) / kSimBlock;
+    if (blockIdx.x == 0 && threadIdx.x < 64 && !only_de) {
+        long long s = 0;
+        for (int b = threadIdx.x; b < nb; b += 64) s += D.blk_draw[b];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (threadIdx.x == 0) {
+            D.ctl->draws_dec = (unsigned long long)s;
+            D.ctl->draws_rec = 0ull;
+        }
+    }
+    if (int(blockIdx.x) * kSimBlock >= N) return;
+    const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    const bool valid = i < N;
+    const bool isE = i < ne;
+    const int li = isE ? i : i - ne;
+    const int q = isE ? -1 : 1;
+    int site = 0, fs = 0;
+    if (valid) {
+        site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+        fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+    }
+    double de = 0.0;
+    if (valid && D.coulomb) {
+        double sde = 0.0;
+        for (int s = 0; s < D.S; ++s) {
+            const float2 p = D.part[s * D.ld + i];
+            sde += double(p.x) + double(p.y);
+        }
+        double bf, bs;   // bindingPotential (chargeagent.cpp:463-481)
+        if (isE) {
+            bf = D.htype[fs] == kHole ? D.self_int + D.binding : 0.0;
+            bs = D.htype[site] == kHole ? D.self_int + D.binding : 0.0;
+        } else {
+            bf = D.etype[fs] == kElectron ? D.self_int - D.binding : 0.0;
+            bs = D.etype[site] == kElectron ? D.self_int - D.binding : 0.0;
+        }
+        de = q * (D.pref * sde - D.self_int * q + bf - bs);
+    }
+    if (valid) D.de[i] = de;
+    if (only_de) return;
+
+    if (threadIdx.x < 4) { att[threadIdx.x] = 0; suc[threadIdx.x] = 0; }
+    const long long before = wg_prefix(D.blk_draw, blockIdx.x, 1, 0, &pre);
+    int total;
+    const int draw = valid ? D.flag[i] : 0;
+    const int off = wg_scan<kSimBlock>(draw, wsum, total);
+    int code = 0, keep = 1;
+    if (valid) {
+        const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
+        if (draw) {
+            const double u = u01(rng_word(D, C.rng_pos + (unsigned long long)(before + off)));
+            if (tf == kEmpty) {
+                double pd = D.vbg[fs] - D.vbg[site];
+                pd *= q;
+                pd += de;
+                int x1, y1, z1, x2, y2, z2;
+                decode(D.d, site, x1, y1, z1);
+                decode(D.d, fs, x2, y2, z2);
+                const int h1 = D.hop + 1;
+                const double cpl = D.coupling[(abs(x1 - x2) * h1 + abs(y1 - y2)) * h1 + abs(z1 - z2)];
+                // metropolisWithCoupling (rand.cpp:91-106)
+                const bool acc = (pd > 0.0) ? (cpl * exp(-pd * D.beta) > u) : (cpl > u);
+                code = acc ? 1 : 0;
+            } else {   // drain: DrainAgent::tryToAccept -> chooseYes(rate) (drainagent.cpp:83-92)
+                const int* ds = isE ? D.e_drain : D.h_drain;
+                const int k = (fs == ds[0]) ? 0 : 1;
+                const int flux = isE ? (LMG_FLUX_EDL + k) : (LMG_FLUX_HDL + k);
+                const int slot = (isE ? 0 : 2) + k;
+                atomicAdd(&att[slot], 1u);
+                if (D.rate[flux] > u) {
+                    atomicAdd(&suc[slot], 1u);
+                    code = 2;
+                }
+            }
+        }
+        int* life = isE ? D.e_life[cur] : D.h_life[cur];
+        int* path = isE ? D.e_path[cur] : D.h_path[cur];
+        life[li] += 1;
+        if (code) path[li] += 1;
+        else fs = site;
+        if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+        D.code[i] = code;
+        D.removed[i] = 0;
+        keep = (code != 2);
+        if (code == 1) {
+            const unsigned long long key = ((unsigned long long)(uint32_t)(C.step + 1) << 32) |
+                                           (unsigned long long)(0xffffffffu - uint32_t(li));
+            atomicMax(isE ? &D.claim_e[fs] : &D.claim_h[fs], key);
+        }
+    }
+    const int ke = __syncthreads_count(valid && isE && keep);
+    const int kh = __syncthreads_count(valid && !isE && keep);
+    if (threadIdx.x == 0) {
+        D.blk_keep[2 * blockIdx.x] = ke;
+        D.blk_keep[2 * blockIdx.x + 1] = kh;
+    }
+    if (threadIdx.x < 4) {
+        const int flux = threadIdx.x < 2 ? LMG_FLUX_EDL + threadIdx.x : LMG_FLUX_HDL + (threadIdx.x - 2);
+        if (att[threadIdx.x]) atomicAdd(&D.ctl->att[flux], (unsigned long long)att[threadIdx.x]);
+        if (suc[threadIdx.x]) atomicAdd(&D.ctl->suc[flux], (unsigned long long)suc[threadIdx.x]);
+    }
+}
+
+// ===========================================================================
+// Serial helpers of the single-workgroup phases (thread 0 owns the stream)
+// ===========================================================================
+struct Serial {
+    const SimDev& D;
+    unsigned long long& pos;
+    __device__ uint32_t word() {
+        if (pos + 624ull >= D.ctl->gen_pos) set_error(D, kErrRing);
+        return rng_word(D, pos++);
+    }
+    __device__ double random() { return u01(word()); }
+    __device__ int integer(int lo, int hi) {      // Boost uniform_int (rand.cpp:68-73)
+        const uint32_t range = uint32_t(hi) - uint32_t(lo);
+        if (range == 0) return lo;
+        if (range == 0xffffffffu) return int(uint32_t(lo) + word());
+        const uint32_t b = uint_bucket(range);
+        for (;;) {
+            uint32_t r = word() / b;
+            if (r <= range) return int(uint32_t(lo) + r);
+        }
+    }
+    __device__ bool metropolis_coupling(double de, double beta, double cpl) {   // rand.cpp:91-106
+        const double u = random();
+        if (de > 0.0) return cpl * exp(-de * beta) > u;
+        return cpl > u;
+    }
+    __device__ bool choose_yes(double p) { return p > random(); }   // rand.cpp:117-124
+};
+
+struct OneShared {
+    int wsum[16];
+    unsigned long long pos;
+    int n_e, n_h, cur;
+    int first_bad;
+    int site, valid, ok, need_energy;
+    int bal_go, bal_src;
+    unsigned int att[10], suc[10];
+    double red[16];
+    uint32_t tail[1024];
+};
+
+__device__ double wg_sum_d(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[w];   // fixed order
+    __syncthreads();
+    return t;
+}
+
+// ===========================================================================
+// recomb_kernel (one workgroup): RecombinationAgent::tryToAccept over the
+// electrons in list order (drainagent.cpp:113-214); sequential through
+// other->removed(), so thread 0 walks the list.  Re-counts the kept carriers.
+// ===========================================================================
+__global__ __launch_bounds__(kOneBlock) void recomb_kernel(SimDev D) {
+    __shared__ OneShared sh;
+    SimCtl& C = *D.ctl;
+    const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
+    const unsigned long long start = C.rng_pos + C.draws_dec;
+    for (int j = threadIdx.x; j < nh; j += blockDim.x) D.h_at[D.h_site[cur][j]] = j;
+    for (int b = threadIdx.x; b < 2 * D.nblk; b += blockDim.x) D.blk_keep[b] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sh.pos = start;
+        Serial R{D, sh.pos};
+        int nb[40];
+        int cand[34];
+        unsigned long long att = 0, suc = 0;
+        for (int i = 0; i < ne; ++i) {
+            const int site = D.e_site[cur][i];
+            int n = 0;
+            if (D.htype[site] == kHole) nb[n++] = site;
+            if (D.rec_range != 0) {
+                const int m = nbr_list(D.d, D.rec_range, site, D.e_drain[0], D.e_drain[1], cand);
+                for (int k = 0; k < m; ++k)
+                    if (D.htype[cand[k]] == kHole) nb[n++] = cand[k];
+            }
+            if (n == 0) continue;
+            const int rs = nb[R.integer(0, n - 1)];
+            const int other = D.h_at[rs];
+            if (D.removed[ne + other]) continue;
+            att += 1;
+            if (R.choose_yes(D.rate[LMG_FLUX_XD])) {
+                suc += 1;
+                D.removed[i] = 1;
+                D.removed[ne + other] = 1;
+            }
+        }
+        C.att[LMG_FLUX_XD] += att;
+        C.suc[LMG_FLUX_XD] += suc;
+        C.draws_rec = sh.pos - start;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int keep = !(D.removed[i] || D.code[i] == 2);
+        if (keep) atomicAdd(&D.blk_keep[2 * (i / kSimBlock) + (i < ne ? 0 : 1)], 1);
+    }
+}
+
+// ===========================================================================
+// tick_kernel: nextTick / completeTick (simulation.cpp:355-414,
+// chargeagent.cpp:163-199).  Among movers claiming one empty site the first in
+// list order wins (the later ones find it occupied and abort); drained and
+// recombined carriers leave; survivors are compacted in order.
+// ===========================================================================
+__global__ __launch_bounds__(kSimBlock) void tick_kernel(SimDev D) {
+    __shared__ int wsum[4];
+    __shared__ long long pre_e, pre_h;
+    const SimCtl& C = *D.ctl;
+    const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur, nxt = cur ^ 1;
+    const int nb = (N + kSimBlock - 1) / kSimBlock;
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        long long se = 0, sh_ = 0;
+        for (int b = threadIdx.x; b < nb; b += 64) {
+            se += D.blk_keep[2 * b];
+            sh_ += D.blk_keep[2 * b + 1];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            se += __shfl_down(se, o, 64);
+            sh_ += __shfl_down(sh_, o, 64);
+        }
+        if (threadIdx.x == 0) {
+            D.ctl->n_e_next = int(se);
+            D.ctl->n_h_next = int(sh_);
+        }
+    }
+    if (int(blockIdx.x) * kSimBlock >= N) return;
+    const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    const bool valid = i < N;
+    const bool isE = i < ne;
+    const int li = isE ? i : i - ne;
+    const long long base_e = wg_prefix(D.blk_keep, blockIdx.x, 2, 0, &pre_e);
+    const long long base_h = wg_prefix(D.blk_keep, blockIdx.x, 2, 1, &pre_h);
+    int keep_e = 0, keep_h = 0, dest = 0;
+    if (valid) {
+        const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+        const int code = D.code[i];
+        const bool rem = D.removed[i] != 0;
+        uint8_t* ty = isE ? D.etype : D.htype;
+        dest = site;
+        if (rem || code == 2) {
+            ty[site] = kEmpty;
+        } else if (code == 1) {
+            const int fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+            const unsigned long long key = ((unsigned long long)(uint32_t)(C.step + 1) << 32) |
+                                           (unsigned long long)(0xffffffffu - uint32_t(li));
+            if ((isE ? D.claim_e[fs] : D.claim_h[fs]) == key) {
+                ty[site] = kEmpty;
+                ty[fs] = isE ? kElectron : kHole;
+                dest = fs;
+            }
+        }
+        const int keep = !(rem || code == 2);
+        keep_e = isE && keep;
+        keep_h = !isE && keep;
+    }
+    int te, th;
+    const int oe = wg_scan<kSimBlock>(keep_e, wsum, te);
+    const int oh = wg_scan<kSimBlock>(keep_h, wsum, th);
+    if (keep_e) {
+        const int w = int(base_e) + oe;
+        D.e_site[nxt][w] = dest;
+        D.e_life[nxt][w] = D.e_life[cur][li];
+        D.e_path[nxt][w] = D.e_path[cur][li];
+    } else if (keep_h) {
+        const int w = int(base_h) + oh;
+        D.h_site[nxt][w] = dest;
+        D.h_life[nxt][w] = D.h_life[cur][li];
+        D.h_path[nxt][w] = D.h_path[cur][li];
+    }
+}
+
+// ===========================================================================
+// Injection (simulation.cpp:233-317, sourceagent.cpp:92-247) and proposals.
+// ===========================================================================
+__device__ void append_carrier(const SimDev& D, OneShared& sh, bool isE, int site) {
+    const int cur = sh.cur;
+    if (isE) {
+        const int n = sh.n_e;
+        if (n >= D.cap_e) { set_error(D, kErrCapacity); return; }
+        D.e_site[cur][n] = site; D.e_life[cur][n] = 0; D.e_path[cur][n] = 0;
+        D.etype[site] = kElectron;
+        sh.n_e = n + 1;
+    } else {
+        const int n = sh.n_h;
+        if (n >= D.cap_h) { set_error(D, kErrCapacity); return; }
+        D.h_site[cur][n] = site; D.h_life[cur][n] = 0; D.h_path[cur][n] = 0;
+        D.htype[site] = kHole;
+        sh.n_h = n + 1;
+    }
+}
+
+// potential of all charges at one site, direct minus left image, times the
+// prefactor: the sum SourceAgent::energyChange adds for source.coulomb
+// (sourceagent.cpp:125-161; coulombE/H/D + coulombImageE/H/D)
+__device__ double wg_site_potential(const SimDev& D, OneShared& sh, int site) {
+    const int ne = sh.n_e, nh = sh.n_h, cur = sh.cur, nd = D.n_d_eff;
+    int x, y, z;
+    decode(D.d, site, x, y, z);
+    const float fx = float(x), fy = float(y), fz = float(z);
+    float s = 0.f, c = 0.f;
+    for (int j = threadIdx.x; j < nd + ne + nh; j += blockDim.x) {
+        float4 q;
+        if (j < nd) q = D.chg[j];
+        else if (j < nd + ne) q = site_f4(D.d, D.e_site[cur][j - nd], -1.f);
+        else q = site_f4(D.d, D.h_site[cur][j - nd - ne], 1.f);
+        const float dy = fy - q.y, dz = fz - q.z;
+        const float dx = fx - q.x, dxi = fx + q.x + 1.f;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float ri = dxi * dxi + dy * dy + dz * dz;
+        const float t = (cut_in(r2, D.c2m1) ? rsqf(r2) : 0.f) - (cut_in(ri, D.c2m1) ? rsqf(ri) : 0.f);
+        kahan(s, c, q.w * t);
+    }
+    return D.pref * wg_sum_d(double(s) - double(c), sh.red);
+}
+
+__device__ bool inject_face(const SimDev& D, OneShared& sh, int flux) {
+    const bool isE = (flux == LMG_FLUX_ESL || flux == LMG_FLUX_ESR);
+    const bool left = (flux == LMG_FLUX_ESL || flux == LMG_FLUX_HSL);
+    if (threadIdx.x == 0) {
+        Serial R{D, sh.pos};
+        sh.att[flux] += 1;
+        // neighborsFace: the sorted x = 0 (or X-1) slice; entry m is X*m (+ X-1)
+        const int m = R.integer(0, D.d.Y * D.d.Z - 1);
+        const int site = D.d.X * m + (left ? 0 : D.d.X - 1);
+        const int n = isE ? sh.n_e : sh.n_h;
+        const int mx = isE ? D.max_e : D.max_h;
+        const uint8_t t = isE ? D.etype[site] : D.htype[site];
+        sh.site = site;
+        sh.valid = !(n >= mx || D.rate[flux] <= 0 || t != kEmpty);   // validToInject
+        sh.need_energy = sh.valid && D.src_metro && D.src_coul;
+    }
+    __syncthreads();
+    double coul = 0.0;
+    if (sh.need_energy) coul = wg_site_potential(D, sh, sh.site);
+    if (threadIdx.x == 0) {
+        sh.ok = 0;
+        if (sh.valid) {
+            Serial R{D, sh.pos};
+            bool go;
+            if (D.src_metro) {
+                const double p2 = D.vbg[sh.site] + coul;   // m_potential of a source is never set: 0
+                const double dE = isE ? (0.0 - p2) : (p2 - 0.0);
+                go = R.metropolis_coupling(dE, D.beta, D.rate[flux]);
+            } else {
+                go = R.choose_yes(D.rate[flux]);
+            }
+            if (go) {
+                append_carrier(D, sh, isE, sh.site);
+                sh.suc[flux] += 1;
+                sh.ok = 1;
+            }
+        }
+    }
+    __syncthreads();
+    return sh.ok != 0;
+}
+
+__device__ void inject_all(const SimDev& D, OneShared& sh) {
+    if (D.sim_type == 1) {
+        if (threadIdx.x == 0) {   // ExcitonSourceAgent::tryToInject
+            Serial R{D, sh.pos};
+            sh.att[LMG_FLUX_XS] += 1;
+            const int site = R.integer(0, D.d.V - 1);
+            const bool valid = !(sh.n_e >= D.max_e || sh.n_h >= D.max_h || D.rate[LMG_FLUX_XS] <= 0 ||
+                                 D.etype[site] != kEmpty || D.htype[site] != kEmpty);
+            if (valid && R.choose_yes(D.rate[LMG_FLUX_XS])) {
+                append_carrier(D, sh, true, site);
+                append_carrier(D, sh, false, site);
+                sh.suc[LMG_FLUX_XS] += 1;
+            }
+        }
+        __syncthreads();
+        if (D.balance) {   // Simulation::balanceCharges (simulation.cpp:253-317)
+            for (int pass = 0; pass < 2; ++pass) {
+                int tries = 0;
+                for (;;) {
+                    if (threadIdx.x == 0) {
+                        const int ne = sh.n_e, nh = sh.n_h;
+                        const bool go = pass == 0 ? (ne - nh < 0 && !(ne >= D.max_e))
+                                                  : (nh - ne < 0 && !(nh >= D.max_h));
+                        sh.bal_go = go;
+                        if (go) {
+                            int src;
+                            if (D.vr > D.vl) src = pass == 0 ? LMG_FLUX_ESL : LMG_FLUX_HSR;
+                            else if (D.vl > D.vr) src = pass == 0 ? LMG_FLUX_ESR : LMG_FLUX_HSL;
+                            else {
+                                Serial R{D, sh.pos};
+                                const bool hi = R.random() > 0.5;
+                                src = pass == 0 ? (hi ? LMG_FLUX_ESL : LMG_FLUX_ESR)
+                                                : (hi ? LMG_FLUX_HSR : LMG_FLUX_HSL);
+                            }
+                            sh.bal_src = src;
+                        }
+                    }
+                    __syncthreads();
+                    if (!sh.bal_go) break;
+                    inject_face(D, sh, sh.bal_src);
+                    if (++tries > 1024) {
+                        if (threadIdx.x == 0) set_error(D, kErrBalance);
+                        break;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    } else {
+        inject_face(D, sh, LMG_FLUX_ESL);
+        inject_face(D, sh, LMG_FLUX_ESR);
+        inject_face(D, sh, LMG_FLUX_HSL);
+        inject_face(D, sh, LMG_FLUX_HSR);
+    }
+}
+
+// chooseFuture (chargeagent.cpp:73-78): neighbour k = integer(0, n-1)
+__device__ int proposal_site(const SimDev& D, bool isE, int site, int k) {
+    const int dl = isE ? D.e_drain[0] : D.h_drain[0];
+    const int dr = isE ? D.e_drain[1] : D.h_drain[1];
+    if (D.hop == 1) {
+        int x, y, z;
+        decode(D.d, site, x, y, z);
+        return nbr1_at(D.d, x, y, z, dl, dr, k);
+    }
+    int nb[40];
+    nbr_list(D.d, D.hop, site, dl, dr, nb);
+    return nb[k];
+}
+__device__ int proposal_count(const SimDev& D, bool isE, int site) {
+    const int dl = isE ? D.e_drain[0] : D.h_drain[0];
+    const int dr = isE ? D.e_drain[1] : D.h_drain[1];
+    if (D.hop == 1) {
+        int x, y, z;
+        decode(D.d, site, x, y, z);
+        return nbr1_count(D.d, x, y, z, dl, dr);
+    }
+    int nb[40];
+    return nbr_list(D.d, D.hop, site, dl, dr, nb);
+}
+
+__device__ void stage_carrier(const SimDev& D, int nd, int i, bool isE, int site, int fs) {
+    const float q = isE ? -1.f : 1.f;
+    const float4 p1 = site_f4(D.d, site, q);
+    D.tgt1[i] = p1;
+    D.tgt2[i] = site_f4(D.d, fs, q);
+    D.chg[nd + i] = p1;
+}
+
+// proposals of every carrier, electrons then holes, from engine output sh.pos
+__device__ void propose_all(const SimDev& D, OneShared& sh) {
+    const int ne = sh.n_e, nh = sh.n_h, N = ne + nh, cur = sh.cur, nd = D.n_d_eff;
+    if (threadIdx.x == 0) sh.first_bad = 0x7fffffff;
+    __syncthreads();
+    const unsigned long long base = sh.pos;
+    if (D.uniform) {
+        // every carrier has >= 2 neighbours (drains on both x faces): carrier i
+        // uses word base + i unless a Boost rejection happened earlier
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const bool isE = i < ne;
+            const int li = isE ? i : i - ne;
+            const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+            if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); continue; }
+            const int n = proposal_count(D, isE, site);
+            const uint32_t range = uint32_t(n - 1);
+            const uint32_t r = rng_word(D, base + i) / uint_bucket(range);
+            if (r > range) atomicMin(&sh.first_bad, i);
+            const int fs = proposal_site(D, isE, site, min(int(r), n - 1));
+            if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+            stage_carrier(D, nd, i, isE, site, fs);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (base + N + 624ull > D.ctl->gen_pos) set_error(D, kErrRing);
+            unsigned long long end = base + N;
+            if (sh.first_bad < N) {   // serial from the first rejecting carrier
+                unsigned long long pos = base + sh.first_bad;
+                Serial R{D, pos};
+                for (int i = sh.first_bad; i < N; ++i) {
+                    const bool isE = i < ne;
+                    const int li = isE ? i : i - ne;
+                    const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+                    const int k = R.integer(0, proposal_count(D, isE, site) - 1);
+                    const int fs = proposal_site(D, isE, site, k);
+                    if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+                    D.tgt2[i] = site_f4(D.d, fs, isE ? -1.f : 1.f);
+                }
+                end = pos;
+            }
+            sh.pos = end;
+        }
+    } else {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const bool isE = i < ne;
+            const int li = isE ? i : i - ne;
+            const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+            if (site < 0 || site >= D.d.V) set_error(D, kErrSite);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {   // some list is shorter than two: serial walk
+            Serial R{D, sh.pos};
+            for (int i = 0; i < N; ++i) {
+                const bool isE = i < ne;
+                const int li = isE ? i : i - ne;
+                const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+                if (site < 0 || site >= D.d.V) continue;
+                const int k = R.integer(0, proposal_count(D, isE, site) - 1);
+                const int fs = proposal_site(D, isE, site, k);
+                if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+                stage_carrier(D, nd, i, isE, site, fs);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// mode 0: finish the pending step (after tick_kernel) and propose the next
+// mode 1: refill and propose only (after a state load)
+__global__ __launch_bounds__(kOneBlock) void inject_kernel(SimDev D, int mode) {
+    __shared__ OneShared sh;
+    SimCtl& C = *D.ctl;
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            sh.n_e = C.n_e_next;
+            sh.n_h = C.n_h_next;
+            sh.cur = C.cur ^ 1;
+            sh.pos = C.rng_pos + C.draws_dec + C.draws_rec;
+        } else {
+            sh.n_e = C.n_e;
+            sh.n_h = C.n_h;
+            sh.cur = C.cur;
+            sh.pos = C.rng_pos;
+        }
+    }
+    if (threadIdx.x < 10) {
+        sh.att[threadIdx.x] = 0;
+        sh.suc[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    if (mode == 1) {
+        mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
+        __syncthreads();
+    } else {
+        inject_all(D, sh);
+    }
+    if (threadIdx.x == 0) C.rng_step0 = sh.pos;
+    propose_all(D, sh);
+    if (threadIdx.x == 0) {
+        C.n_e = sh.n_e;
+        C.n_h = sh.n_h;
+        C.cur = sh.cur;
+        C.rng_pos = sh.pos;
+        C.counts[0] = sh.n_e + sh.n_h;
+        C.counts[1] = D.n_d_eff + sh.n_e + sh.n_h;
+        if (mode == 0) C.step += 1;
+    }
+    if (threadIdx.x < 10 && mode == 0) {
+        C.att[threadIdx.x] += sh.att[threadIdx.x];
+        C.suc[threadIdx.x] += sh.suc[threadIdx.x];
+    }
+}
+
+}  // namespace lmg

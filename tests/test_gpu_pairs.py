@@ -1,8 +1,10 @@
 """Device pair sums and potential map against the FP64 oracle.
 
-Tolerances (stated): pair potentials and maps are FP32 terms with compensated
+Tolerance (stated): pair potentials and maps are FP32 terms with compensated
 accumulation; every output agrees with the FP64 oracle to
-|gpu - ref| <= 1e-5 |ref| + 1e-6 max|ref|.
+    |gpu - ref| <= 1e-5 |ref| + 5e-8 S,   S = pref * sum_j |q_j| f(r_j)
+(S, the sum of the magnitudes of the terms, is the condition scale of a sum with
+cancellation; 5e-8 is 0.84 FP32 unit roundoff).
 """
 import numpy as np
 import pytest
@@ -14,12 +16,14 @@ pytestmark = pytest.mark.gpu
 PREF = 0.41141839736460467
 
 
-def tol_check(got, ref, rtol=1e-5, scale_atol=1e-6):
+def tol_check(got, ref, sabs):
     err = np.abs(got - ref)
-    bound = rtol * np.abs(ref) + scale_atol * max(np.abs(ref).max(), 1e-30)
+    bound = 1e-5 * np.abs(ref) + 5e-8 * np.abs(sabs) + 1e-15
     bad = np.nonzero(err > bound)[0]
-    assert len(bad) == 0, f"{len(bad)} outputs out of tolerance, worst {err.max():.3e} at {bad[:5]}"
-    return float(err.max()), float(np.sqrt(np.mean((err / max(np.abs(ref).max(), 1e-30)) ** 2)))
+    assert len(bad) == 0, (f"{len(bad)} outputs out of tolerance, worst {err.max():.3e}, "
+                           f"worst ratio {np.max(err / bound):.2f}")
+    rel = err / np.maximum(np.abs(ref), 1e-30)
+    return float(err.max()), float(np.sqrt(np.mean(rel ** 2))), float(np.max(err / bound))
 
 
 @pytest.mark.parametrize("dims,n,nt,cutoff,flags,sigma", [
@@ -38,7 +42,8 @@ def test_pair_potential_matches_oracle(gpu, oracle, dims, n, nt, cutoff, flags, 
     ctx = gpu.Context(dims, cutoff=cutoff, gaussian_sigma=sigma)
     got = ctx.pair_potential(cs, cq, ts, flags)
     ref = oracle.pair_potential(dims, cutoff, cs, cq, ts, flags=flags, sigma=sigma)
-    tol_check(got, ref)
+    sabs = oracle.pair_potential(dims, cutoff, cs, np.abs(cq), ts, flags=flags, sigma=sigma)
+    tol_check(got, ref, sabs)
 
 
 def test_pair_potential_special_slot_targets(gpu, oracle):
@@ -50,7 +55,8 @@ def test_pair_potential_special_slot_targets(gpu, oracle):
     cq = rng.choice([-1, 1], 60)
     ts = np.array([V + 2, V + 3, V + 4, 0, 5], np.int32)
     ctx = gpu.Context(dims, cutoff=50)
-    tol_check(ctx.pair_potential(cs, cq, ts), oracle.pair_potential(dims, 50, cs, cq, ts))
+    tol_check(ctx.pair_potential(cs, cq, ts), oracle.pair_potential(dims, 50, cs, cq, ts),
+              oracle.pair_potential(dims, 50, cs, np.abs(cq), ts))
 
 
 def test_pair_potential_edge_cases(gpu):
@@ -105,9 +111,10 @@ def test_potential_map_matches_oracle(gpu, oracle, dims, n, cutoff, flags, sigma
     cq = rng.choice([-1, 1], n).astype(np.int32)
     ctx = gpu.Context(dims, cutoff=max(cutoff, 1), gaussian_sigma=sigma)
     got = ctx.potential_map(cs, cq, flags).astype(np.float64)
-    ref = oracle.potential_map(dims, cutoff if not (flags & 4) else 0, cs, cq,
-                               flags=flags & 1, sigma=sigma, threads=8)
-    tol_check(got, ref)
+    cut = cutoff if not (flags & 4) else 0
+    ref = oracle.potential_map(dims, cut, cs, cq, flags=flags & 1, sigma=sigma, threads=8)
+    sabs = oracle.potential_map(dims, cut, cs, np.abs(cq), flags=flags & 1, sigma=sigma, threads=8)
+    tol_check(got, ref, sabs)
 
 
 @pytest.mark.parametrize("cutoff,flags", [(50, 0), (50, 4)])
@@ -121,6 +128,9 @@ def test_potential_map_full_size_sampled(gpu, oracle, cutoff, flags):
     ctx = gpu.Context(dims, cutoff=cutoff)
     got = ctx.potential_map(cs, cq, flags)
     sample = rng.choice(V, 4096, replace=False)
-    ref = oracle.pair_potential(dims, cutoff if not (flags & 4) else 100000, cs, cq, sample)
-    mx, rms = tol_check(got[sample].astype(np.float64), ref)
-    print(f"map 256x256x64 n=50000 flags={flags}: max abs err {mx:.3e}, rms rel err {rms:.3e}")
+    cut = cutoff if not (flags & 4) else 100000
+    ref = oracle.pair_potential(dims, cut, cs, cq, sample)
+    sabs = oracle.pair_potential(dims, cut, cs, np.abs(cq), sample)
+    mx, rms, ratio = tol_check(got[sample].astype(np.float64), ref, sabs)
+    print(f"map 256x256x64 n=50000 flags={flags}: max abs err {mx:.3e}, rms rel err {rms:.3e}, "
+          f"worst err/bound {ratio:.2f}")
