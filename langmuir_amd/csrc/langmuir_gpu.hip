@@ -199,6 +199,7 @@ struct lmg_ctx {
     bool sync_mode = false;
     lmg_sim_config cfg{};
     SimDev sim{};
+    SimDev* d_sim = nullptr;   // device copy the kernels read
     SimCtl* ctl = nullptr;
     std::vector<void*> sim_allocs;
     std::vector<hipEvent_t> events;
@@ -320,6 +321,7 @@ void sim_free(lmg_ctx* c) {
     c->sim_allocs.clear();
     c->sim_loaded = false;
     c->ctl = nullptr;
+    c->d_sim = nullptr;
 }
 
 template <class T>
@@ -341,24 +343,37 @@ unsigned long long next_pow2(unsigned long long v) {
 // Coulomb off only the flags and the refill run)
 lmg_status sim_launch_de(lmg_ctx* c) {
     const SimDev& D = c->sim;
+    const SimDev* P = c->d_sim;
     if (!D.coulomb)
-        hipLaunchKernelGGL((de_kernel<false, false>), dim3(D.nblk + 1, 1), dim3(kSimBlock), 0, c->stream, D);
+        hipLaunchKernelGGL((de_kernel<false, false>), dim3(D.nblk + 1, 1), dim3(kSimBlock), 0, c->stream, P);
     else if (D.gauss)
-        hipLaunchKernelGGL((de_kernel<true, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, D);
+        hipLaunchKernelGGL((de_kernel<true, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
     else
-        hipLaunchKernelGGL((de_kernel<false, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, D);
+        hipLaunchKernelGGL((de_kernel<false, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
     return launch_check(c, "de_kernel");
+}
+
+// proposals of the next step (inject_kernel makes them itself when some site
+// has fewer than two neighbours)
+lmg_status sim_launch_propose(lmg_ctx* c) {
+    const SimDev& D = c->sim;
+    if (D.uniform) {
+        hipLaunchKernelGGL(propose_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
+        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(64), 0, c->stream, c->d_sim);
+    }
+    return launch_check(c, "step kernels");
 }
 
 // decisions, recombination, completeTick/compaction, injection + proposals
 lmg_status sim_launch_commit(lmg_ctx* c) {
     const SimDev& D = c->sim;
-    hipLaunchKernelGGL(decide_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, D, 0);
+    const SimDev* P = c->d_sim;
+    hipLaunchKernelGGL(decide_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, P, 0);
     if (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0)
-        hipLaunchKernelGGL(recomb_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, D);
-    hipLaunchKernelGGL(tick_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, D);
-    hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, D, 0);
-    return launch_check(c, "commit kernels");
+        hipLaunchKernelGGL(recomb_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P);
+    hipLaunchKernelGGL(tick_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, P);
+    hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P, 0);
+    return sim_launch_propose(c);
 }
 
 }  // namespace
@@ -395,7 +410,7 @@ lmg_status lmg_create(const lmg_params* params, lmg_ctx** out) {
     if (p.device_id < 0 || p.device_id >= n) return fail(nullptr, LMG_ERR_ARG, "device_id out of range");
     lmg_ctx* c = new lmg_ctx;
     c->p = p;
-    c->d = Dims{p.grid_x, p.grid_y, p.grid_z, p.grid_x * p.grid_y, p.grid_x * p.grid_y * p.grid_z};
+    c->d = make_dims(p.grid_x, p.grid_y, p.grid_z);
     c->dev = p.device_id;
     hipError_t e = hipSetDevice(c->dev);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -609,9 +624,11 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         (st = sim_alloc(c, D.tgt1, cap)) || (st = sim_alloc(c, D.tgt2, cap)) ||
         (st = sim_alloc(c, D.chg, cap + D.n_d_eff)) || (st = sim_alloc(c, D.part, size_t(D.S) * D.ld)) ||
         (st = sim_alloc(c, D.blk_draw, D.nblk)) || (st = sim_alloc(c, D.blk_keep, 2 * D.nblk)) ||
-        (st = sim_alloc(c, D.de, cap)) || (st = sim_alloc(c, D.ring, ring)) || (st = sim_alloc(c, c->ctl, 1)))
+        (st = sim_alloc(c, D.de, cap)) || (st = sim_alloc(c, D.ring, ring)) || (st = sim_alloc(c, c->ctl, 1)) ||
+        (st = sim_alloc(c, c->d_sim, 1)))
         return st;
     D.ctl = c->ctl;
+    HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
 
     // uploads
     HIPCHK(c, hipMemcpy(D.etype, et.data(), Vt, hipMemcpyHostToDevice));
@@ -656,8 +673,8 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     C.step = s->current_step;
     HIPCHK(c, hipMemcpy(c->ctl, &C, sizeof(C), hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
-    hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, D, 1);
-    if ((st = launch_check(c, "inject_kernel(prepare)")) != LMG_OK) return st;
+    hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim, 1);
+    if ((st = sim_launch_propose(c)) != LMG_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->sim_loaded = true;
     int32_t code = 0;
@@ -779,7 +796,7 @@ lmg_status lmg_sim_delta_e(lmg_ctx* c, int32_t* fsite_e, int32_t* fsite_h, doubl
     lmg_status st;
     if ((st = use_device(c)) != LMG_OK) return st;
     if ((st = sim_launch_de(c)) != LMG_OK) return st;
-    hipLaunchKernelGGL(decide_kernel, dim3(c->sim.nblk), dim3(kSimBlock), 0, c->stream, c->sim, 1);
+    hipLaunchKernelGGL(decide_kernel, dim3(c->sim.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim, 1);
     if ((st = launch_check(c, "decide_kernel(dE)")) != LMG_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     SimCtl C;

@@ -15,14 +15,29 @@ constexpr uint8_t kEmpty = 0, kElectron = 1, kHole = 2, kDefect = 3, kSource = 4
 
 struct Dims {
     int X, Y, Z, XY, V;
+    double inv_x, inv_xy;   // 1/X and 1/XY for the division-free decode
 };
+
+inline Dims make_dims(int X, int Y, int Z) {
+    return Dims{X, Y, Z, X * Y, X * Y * Z, 1.0 / double(X), 1.0 / double(X * Y)};
+}
+
+// floor(n / d) for 0 <= n < 2^31: FP64 estimate (off by at most one) + correction
+__host__ __device__ inline int div_fast(int n, int d, double inv) {
+    int q = int(double(n) * inv);
+    int r = n - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
 
 // cubicgrid.cpp:132-145: x = s % X, y = s / X - (s / XY) * Y, z = s / XY.  Also
 // used on special slots s >= V, exactly like kernel.cl:229-231.
 __host__ __device__ inline void decode(const Dims& d, int s, int& x, int& y, int& z) {
-    z = s / d.XY;
-    y = s / d.X - z * d.Y;
-    x = s % d.X;
+    z = div_fast(s, d.XY, d.inv_xy);
+    const int r = s - z * d.XY;          // s/X - z*Y == (s - z*XY)/X for s >= 0
+    y = div_fast(r, d.X, d.inv_x);
+    x = r - y * d.X;
 }
 __host__ __device__ inline int encode(const Dims& d, int x, int y, int z) {
     return d.X * (y + z * d.Y) + x;   // cubicgrid.cpp:162-165
@@ -80,12 +95,29 @@ __device__ inline int nbr1_at(const Dims& d, int x, int y, int z, int dl, int dr
     return dr;
 }
 
-// General list for hopping.range 1 or 2; out needs 34 entries.
-__device__ inline int nbr_list(const Dims& d, int h, int site, int dl, int dr, int* out) {
+// k-th range-1 neighbour with its coordinates (drain slots decode like the
+// reference's kernel.cl:229-231)
+__device__ inline int nbr1_pick(const Dims& d, int x, int y, int z, int dl, int dr, int k, int& fx, int& fy,
+                                int& fz) {
+    fx = x; fy = y; fz = z;
+    if (x > 0) { if (k == 0) { fx = x - 1; return encode(d, fx, y, z); } --k; }
+    if (x < d.X - 1) { if (k == 0) { fx = x + 1; return encode(d, fx, y, z); } --k; }
+    if (y > 0) { if (k == 0) { fy = y - 1; return encode(d, x, fy, z); } --k; }
+    if (y < d.Y - 1) { if (k == 0) { fy = y + 1; return encode(d, x, fy, z); } --k; }
+    if (z > 0) { if (k == 0) { fz = z - 1; return encode(d, x, y, fz); } --k; }
+    if (z < d.Z - 1) { if (k == 0) { fz = z + 1; return encode(d, x, y, fz); } --k; }
+    const int s = (x == 0 && dl >= 0 && k == 0) ? dl : dr;
+    decode(d, s, fx, fy, fz);
+    return s;
+}
+
+// Calls emit(site) for every neighbour in list order, hopping.range 1 or 2,
+// without materialising the list (no private arrays, hence no scratch).
+template <class F>
+__device__ inline void nbr_enum(const Dims& d, int h, int site, int dl, int dr, F&& emit) {
     int x, y, z;
     decode(d, site, x, y, z);
-    int n = 0;
-#define LMG_P(a, b, c) (out[n++] = encode(d, (a), (b), (c)))
+#define LMG_P(a, b, c) emit(encode(d, (a), (b), (c)))
     if (h == 1) {
         if (x > 0) LMG_P(x - 1, y, z);
         if (x < d.X - 1) LMG_P(x + 1, y, z);
@@ -156,9 +188,23 @@ __device__ inline int nbr_list(const Dims& d, int h, int site, int dl, int dr, i
         }
     }
 #undef LMG_P
-    if (x == 0 && dl >= 0) out[n++] = dl;        // cubicgrid.cpp:447-457
-    if (x == d.X - 1 && dr >= 0) out[n++] = dr;  // cubicgrid.cpp:459-468
+    if (x == 0 && dl >= 0) emit(dl);        // cubicgrid.cpp:447-457
+    if (x == d.X - 1 && dr >= 0) emit(dr);  // cubicgrid.cpp:459-468
+}
+
+__device__ inline int nbr_count(const Dims& d, int h, int site, int dl, int dr) {
+    int n = 0;
+    nbr_enum(d, h, site, dl, dr, [&](int) { ++n; });
     return n;
+}
+
+__device__ inline int nbr_at(const Dims& d, int h, int site, int dl, int dr, int k) {
+    int n = 0, r = -1;
+    nbr_enum(d, h, site, dl, dr, [&](int s) {
+        if (n == k) r = s;
+        ++n;
+    });
+    return r;
 }
 
 }  // namespace lmg

@@ -10,8 +10,9 @@
 //   recomb_kernel  (1 WG)   RecombinationAgent sweep (solar cell, rate > 0)
 //   tick_kernel    (grid)   completeTick conflict rule and order-preserving
 //                           compaction of the lists
-//   inject_kernel  (1 WG)   injection (serial in the reference) and the next
-//                           step's proposals
+//   inject_kernel  (1 WG)   injection (serial in the reference)
+//   propose_kernel (grid)   the next step's proposals, one engine word each
+//   propose_fix_kernel      serial redo after a (rare) Boost rejection
 //
 // Offsets into the random stream are computed so that every draw uses exactly
 // the word the reference's serial loop would use.
@@ -187,8 +188,31 @@ __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsi
 //   1/sqrt(r2) - 1/sqrt(r1) = (r1 - r2) (ab)^2 / (a + b),  a = r1^-1/2, b = r2^-1/2,
 // with r1 - r2 an exact integer.
 // ===========================================================================
+// [0<r2<c2] f(r2) - [0<r1<c2] f(r1) for one charge q, target pair (p1, p1+h)
+template <bool kGauss>
+__device__ __forceinline__ float de_term(const float4& p1, const float4& q, float hx, float hy, float hz,
+                                         float hh, uint32_t c2m1, float gk) {
+    const float dx = p1.x - q.x, dy = p1.y - q.y, dz = p1.z - q.z;
+    const float r1 = dx * dx + dy * dy + dz * dz;
+    // |p2 - q|^2 = r1 + t with t = 2 h.(p1 - q) + |h|^2, exact in integers
+    const float t = fmaf(2.f, hx * dx + hy * dy + hz * dz, hh);
+    const float r2 = r1 + t;
+    const bool in1 = cut_in(r1, c2m1), in2 = cut_in(r2, c2m1);
+    const float a = rsqf(r1), b = rsqf(r2);
+    if (kGauss) {
+        const float fa = erff(gk * (r1 * a)) * a;
+        const float fb = erff(gk * (r2 * b)) * b;
+        return (in2 ? fb : 0.f) - (in1 ? fa : 0.f);
+    }
+    // b - a = (r1 - r2) / (sqrt(r1) sqrt(r2) (sqrt(r1) + sqrt(r2))) = -t / (r1 r2 (a + b))
+    const float both = -t * __builtin_amdgcn_rcpf((r1 * r2) * (a + b));
+    const float one = (in2 ? b : 0.f) - (in1 ? a : 0.f);
+    return (in1 & in2) ? both : one;
+}
+
 template <bool kGauss, bool kPairs>
-__global__ __launch_bounds__(kSimBlock) void de_kernel(SimDev D) {
+__global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
     __shared__ float4 tile[256];
     __shared__ uint32_t tail[1024];
     if (int(blockIdx.x) == D.nblk) {
@@ -226,29 +250,18 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(SimDev D) {
         __syncthreads();
         if (threadIdx.x < n) tile[threadIdx.x] = D.chg[base + threadIdx.x];
         __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += 32) {
-            const int k1 = min(n, k0 + 32);
+        int k0 = 0;
+        for (; k0 + 32 <= n; k0 += 32) {   // full blocks: unrolled, branch-free
             float acc = 0.f;
-            for (int k = k0; k < k1; ++k) {
-                const float4 q = tile[k];
-                const float dx = p1.x - q.x, dy = p1.y - q.y, dz = p1.z - q.z;
-                const float r1 = dx * dx + dy * dy + dz * dz;
-                // |p2 - q|^2 = r1 + 2 h.(p1 - q) + |h|^2, exact in integers
-                const float r2 = r1 + (2.f * (hx * dx + hy * dy + hz * dz) + hh);
-                const bool in1 = cut_in(r1, c2m1), in2 = cut_in(r2, c2m1);
-                const float a = rsqf(r1), b = rsqf(r2);
-                float term;
-                if (kGauss) {
-                    const float fa = in1 ? erff(gk * (r1 * a)) * a : 0.f;
-                    const float fb = in2 ? erff(gk * (r2 * b)) * b : 0.f;
-                    term = fb - fa;
-                } else {
-                    const float ab = a * b;
-                    const float both = (r1 - r2) * (ab * ab) * __builtin_amdgcn_rcpf(a + b);
-                    term = (in1 && in2) ? both : ((in2 ? b : 0.f) - (in1 ? a : 0.f));
-                }
-                acc = fmaf(q.w, term, acc);
-            }
+#pragma unroll 8
+            for (int k = k0; k < k0 + 32; ++k)
+                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
+            kahan(s, c, acc);
+        }
+        if (k0 < n) {
+            float acc = 0.f;
+            for (int k = k0; k < n; ++k)
+                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
             kahan(s, c, acc);
         }
     }
@@ -260,7 +273,8 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(SimDev D) {
 // The draw of carrier i is word rng_pos + (draws of carriers before i).
 // only_de = 1 assembles dE (chargeagent.cpp:215-296) and stops.
 // ===========================================================================
-__global__ __launch_bounds__(kSimBlock) void decide_kernel(SimDev D, int only_de) {
+__global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restrict__ Dp, int only_de) {
+    const SimDev& D = *Dp;
     __shared__ int wsum[4];
     __shared__ long long pre;
     __shared__ unsigned int att[4], suc[4];
@@ -289,9 +303,9 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(SimDev D, int only_de
     }
     double de = 0.0;
     if (valid && D.coulomb) {
-        double sde = 0.0;
-        for (int s = 0; s < D.S; ++s) {
-            const float2 p = D.part[s * D.ld + i];
+        double sde = 0.0;   // the S chunk partials, in chunk order (deterministic)
+        for (int q = 0; q < D.S; ++q) {
+            const float2 p = D.part[q * D.ld + i];
             sde += double(p.x) + double(p.y);
         }
         double bf, bs;   // bindingPotential (chargeagent.cpp:463-481)
@@ -427,7 +441,8 @@ __device__ double wg_sum_d(double v, double* red) {
 // electrons in list order (drainagent.cpp:113-214); sequential through
 // other->removed(), so thread 0 walks the list.  Re-counts the kept carriers.
 // ===========================================================================
-__global__ __launch_bounds__(kOneBlock) void recomb_kernel(SimDev D) {
+__global__ __launch_bounds__(kOneBlock) void recomb_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
     __shared__ OneShared sh;
     SimCtl& C = *D.ctl;
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
@@ -438,20 +453,29 @@ __global__ __launch_bounds__(kOneBlock) void recomb_kernel(SimDev D) {
     if (threadIdx.x == 0) {
         sh.pos = start;
         Serial R{D, sh.pos};
-        int nb[40];
-        int cand[34];
         unsigned long long att = 0, suc = 0;
         for (int i = 0; i < ne; ++i) {
             const int site = D.e_site[cur][i];
-            int n = 0;
-            if (D.htype[site] == kHole) nb[n++] = site;
-            if (D.rec_range != 0) {
-                const int m = nbr_list(D.d, D.rec_range, site, D.e_drain[0], D.e_drain[1], cand);
-                for (int k = 0; k < m; ++k)
-                    if (D.htype[cand[k]] == kHole) nb[n++] = cand[k];
-            }
+            // candidates: the own site, then the neighbour list (range
+            // recombination.range), keeping the sites that hold a hole
+            const bool self = D.htype[site] == kHole;
+            int n = self;
+            if (D.rec_range != 0)
+                nbr_enum(D.d, D.rec_range, site, D.e_drain[0], D.e_drain[1],
+                         [&](int s) { n += (D.htype[s] == kHole); });
             if (n == 0) continue;
-            const int rs = nb[R.integer(0, n - 1)];
+            int k = R.integer(0, n - 1);
+            int rs = site;
+            if (!(self && k == 0)) {
+                k -= self;
+                int m = 0;
+                nbr_enum(D.d, D.rec_range, site, D.e_drain[0], D.e_drain[1], [&](int s) {
+                    if (D.htype[s] == kHole) {
+                        if (m == k) rs = s;
+                        ++m;
+                    }
+                });
+            }
             const int other = D.h_at[rs];
             if (D.removed[ne + other]) continue;
             att += 1;
@@ -478,7 +502,8 @@ __global__ __launch_bounds__(kOneBlock) void recomb_kernel(SimDev D) {
 // list order wins (the later ones find it occupied and abort); drained and
 // recombined carriers leave; survivors are compacted in order.
 // ===========================================================================
-__global__ __launch_bounds__(kSimBlock) void tick_kernel(SimDev D) {
+__global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
     __shared__ int wsum[4];
     __shared__ long long pre_e, pre_h;
     const SimCtl& C = *D.ctl;
@@ -696,9 +721,7 @@ __device__ int proposal_site(const SimDev& D, bool isE, int site, int k) {
         decode(D.d, site, x, y, z);
         return nbr1_at(D.d, x, y, z, dl, dr, k);
     }
-    int nb[40];
-    nbr_list(D.d, D.hop, site, dl, dr, nb);
-    return nb[k];
+    return nbr_at(D.d, D.hop, site, dl, dr, k);
 }
 __device__ int proposal_count(const SimDev& D, bool isE, int site) {
     const int dl = isE ? D.e_drain[0] : D.h_drain[0];
@@ -708,8 +731,7 @@ __device__ int proposal_count(const SimDev& D, bool isE, int site) {
         decode(D.d, site, x, y, z);
         return nbr1_count(D.d, x, y, z, dl, dr);
     }
-    int nb[40];
-    return nbr_list(D.d, D.hop, site, dl, dr, nb);
+    return nbr_count(D.d, D.hop, site, dl, dr);
 }
 
 __device__ void stage_carrier(const SimDev& D, int nd, int i, bool isE, int site, int fs) {
@@ -720,76 +742,59 @@ __device__ void stage_carrier(const SimDev& D, int nd, int i, bool isE, int site
     D.chg[nd + i] = p1;
 }
 
-// proposals of every carrier, electrons then holes, from engine output sh.pos
-__device__ void propose_all(const SimDev& D, OneShared& sh) {
-    const int ne = sh.n_e, nh = sh.n_h, N = ne + nh, cur = sh.cur, nd = D.n_d_eff;
-    if (threadIdx.x == 0) sh.first_bad = 0x7fffffff;
-    __syncthreads();
-    const unsigned long long base = sh.pos;
-    if (D.uniform) {
-        // every carrier has >= 2 neighbours (drains on both x faces): carrier i
-        // uses word base + i unless a Boost rejection happened earlier
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const bool isE = i < ne;
-            const int li = isE ? i : i - ne;
-            const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-            if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); continue; }
-            const int n = proposal_count(D, isE, site);
-            const uint32_t range = uint32_t(n - 1);
-            const uint32_t r = rng_word(D, base + i) / uint_bucket(range);
-            if (r > range) atomicMin(&sh.first_bad, i);
-            const int fs = proposal_site(D, isE, site, min(int(r), n - 1));
-            if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
-            stage_carrier(D, nd, i, isE, site, fs);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if (base + N + 624ull > D.ctl->gen_pos) set_error(D, kErrRing);
-            unsigned long long end = base + N;
-            if (sh.first_bad < N) {   // serial from the first rejecting carrier
-                unsigned long long pos = base + sh.first_bad;
-                Serial R{D, pos};
-                for (int i = sh.first_bad; i < N; ++i) {
-                    const bool isE = i < ne;
-                    const int li = isE ? i : i - ne;
-                    const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-                    const int k = R.integer(0, proposal_count(D, isE, site) - 1);
-                    const int fs = proposal_site(D, isE, site, k);
-                    if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
-                    D.tgt2[i] = site_f4(D.d, fs, isE ? -1.f : 1.f);
-                }
-                end = pos;
-            }
-            sh.pos = end;
-        }
+// chooseFuture for one carrier from engine word w (one decode per carrier);
+// returns false when Boost's uniform_int would reject w
+__device__ __forceinline__ bool propose_one(const SimDev& D, int nd, int i, bool isE, int li, int site,
+                                            uint32_t w) {
+    const int dl = isE ? D.e_drain[0] : D.h_drain[0];
+    const int dr = isE ? D.e_drain[1] : D.h_drain[1];
+    const float q = isE ? -1.f : 1.f;
+    int x, y, z, fx, fy, fz, fs, n;
+    decode(D.d, site, x, y, z);
+    if (D.hop == 1) n = nbr1_count(D.d, x, y, z, dl, dr);
+    else n = nbr_count(D.d, D.hop, site, dl, dr);
+    const uint32_t range = uint32_t(n - 1);
+    const uint32_t r = w / uint_bucket(range);
+    const int k = min(int(r), n - 1);
+    if (D.hop == 1) {
+        fs = nbr1_pick(D.d, x, y, z, dl, dr, k, fx, fy, fz);
     } else {
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
-            const bool isE = i < ne;
-            const int li = isE ? i : i - ne;
-            const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-            if (site < 0 || site >= D.d.V) set_error(D, kErrSite);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {   // some list is shorter than two: serial walk
-            Serial R{D, sh.pos};
-            for (int i = 0; i < N; ++i) {
-                const bool isE = i < ne;
-                const int li = isE ? i : i - ne;
-                const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-                if (site < 0 || site >= D.d.V) continue;
-                const int k = R.integer(0, proposal_count(D, isE, site) - 1);
-                const int fs = proposal_site(D, isE, site, k);
-                if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
-                stage_carrier(D, nd, i, isE, site, fs);
-            }
-        }
+        fs = nbr_at(D.d, D.hop, site, dl, dr, k);
+        decode(D.d, fs, fx, fy, fz);
     }
-    __syncthreads();
+    if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+    const float4 p1 = make_float4(float(x), float(y), float(z), q);
+    D.tgt1[i] = p1;
+    D.tgt2[i] = make_float4(float(fx), float(fy), float(fz), q);
+    D.chg[nd + i] = p1;
+    return r <= range;
 }
 
-// mode 0: finish the pending step (after tick_kernel) and propose the next
-// mode 1: refill and propose only (after a state load)
-__global__ __launch_bounds__(kOneBlock) void inject_kernel(SimDev D, int mode) {
+// Serial chooseFuture from carrier i0 on, starting at engine output pos
+// (thread 0 only): the rare path after a Boost rejection, and lattices where
+// some site has fewer than two neighbours (a one-entry list draws nothing).
+__device__ unsigned long long propose_serial(const SimDev& D, int i0, int ne, int N, int cur,
+                                             unsigned long long pos) {
+    Serial R{D, pos};
+    for (int i = i0; i < N; ++i) {
+        const bool isE = i < ne;
+        const int li = isE ? i : i - ne;
+        const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+        if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); continue; }
+        const int k = R.integer(0, proposal_count(D, isE, site) - 1);
+        const int fs = proposal_site(D, isE, site, k);
+        stage_carrier(D, D.n_d_eff, i, isE, site, fs);
+        if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+    }
+    return pos;
+}
+
+// mode 0: injection for the pending step (after tick_kernel)
+// mode 1: refill the random stream after a state load
+// Leaves the new lists' sizes, the proposal start rng_step0 and counts in ctl;
+// with D.uniform == 0 it also makes the proposals (serially).
+__global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restrict__ Dp, int mode) {
+    const SimDev& D = *Dp;
     __shared__ OneShared sh;
     SimCtl& C = *D.ctl;
     if (threadIdx.x == 0) {
@@ -810,27 +815,56 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(SimDev D, int mode) {
         sh.suc[threadIdx.x] = 0;
     }
     __syncthreads();
-    if (mode == 1) {
-        mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
-        __syncthreads();
-    } else {
-        inject_all(D, sh);
-    }
-    if (threadIdx.x == 0) C.rng_step0 = sh.pos;
-    propose_all(D, sh);
+    if (mode == 1) mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
+    else inject_all(D, sh);
+    __syncthreads();
     if (threadIdx.x == 0) {
+        const int N = sh.n_e + sh.n_h;
         C.n_e = sh.n_e;
         C.n_h = sh.n_h;
         C.cur = sh.cur;
-        C.rng_pos = sh.pos;
-        C.counts[0] = sh.n_e + sh.n_h;
-        C.counts[1] = D.n_d_eff + sh.n_e + sh.n_h;
+        C.rng_step0 = sh.pos;
+        C.first_bad = 0x7fffffff;
+        C.counts[0] = N;
+        C.counts[1] = D.n_d_eff + N;
         if (mode == 0) C.step += 1;
+        if (!D.uniform) C.rng_pos = propose_serial(D, 0, sh.n_e, N, sh.cur, sh.pos);
     }
     if (threadIdx.x < 10 && mode == 0) {
         C.att[threadIdx.x] += sh.att[threadIdx.x];
         C.suc[threadIdx.x] += sh.suc[threadIdx.x];
     }
+}
+
+// chooseFuture for every carrier when every site has >= 2 neighbours (drains
+// on both x faces): carrier i uses engine output rng_step0 + i unless a Boost
+// rejection happened before it; the first rejecting index is recorded.
+__global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
+    const SimCtl& C = *D.ctl;
+    const int ne = C.n_e, N = ne + C.n_h, cur = C.cur;
+    const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    if (i >= N) return;
+    const bool isE = i < ne;
+    const int li = isE ? i : i - ne;
+    const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+    if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); return; }
+    if (!propose_one(D, D.n_d_eff, i, isE, li, site, rng_word(D, C.rng_step0 + (unsigned long long)i)))
+        atomicMin(&D.ctl->first_bad, i);
+}
+
+// After propose_kernel: redo serially from the first rejection (if any) and
+// set the engine position of the pending decisions.
+__global__ void propose_fix_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
+    SimCtl& C = *D.ctl;
+    if (threadIdx.x != 0) return;
+    const int ne = C.n_e, N = ne + C.n_h;
+    const unsigned long long base = C.rng_step0;
+    if (base + N + 624ull > C.gen_pos) set_error(D, kErrRing);
+    const int fb = C.first_bad;
+    C.rng_pos = fb < N ? propose_serial(D, fb, ne, N, C.cur, base + (unsigned long long)fb)
+                       : base + (unsigned long long)N;
 }
 
 }  // namespace lmg
