@@ -179,6 +179,58 @@ def bench_map(reps=3):
     return out
 
 
+BATCH_LATTICES = ((32, 32, 32), (48, 48, 24), (64, 64, 16), (64, 64, 32), (96, 96, 16), (128, 128, 16))
+
+
+def batch_workload(n_rep=256, seed=4):
+    """Independent transistor-type replicas: lattice drawn from 32^3 .. 128x128x16,
+    1-V bias along x, holes N ~ U[100, 5000] (at most 10 % of sites) at random
+    sites, trap fraction ~ U[0, 0.1] with depths clip(N(0.3, 0.1), 0.1, 0.5) eV,
+    cutoff ~ U{20..50}, per-replica seeds."""
+    from langmuir_amd.host import SimulationParameters, background_potential
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(n_rep):
+        X, Y, Z = BATCH_LATTICES[rng.integers(len(BATCH_LATTICES))]
+        V = X * Y * Z
+        n = int(min(rng.integers(100, 5001), V // 10))
+        p = SimulationParameters(
+            grid_x=X, grid_y=Y, grid_z=Z, coulomb_carriers=True, electron_percentage=0.0,
+            hole_percentage=(n + 0.5) / V, seed_charges=1.0, voltage_left=1.0, e_source_l_rate=0.0,
+            h_source_l_rate=0.9, h_drain_r_rate=0.9, electrostatic_cutoff=int(rng.integers(20, 51)),
+            random_seed=int(rng.integers(1, 2 ** 31)))
+        nt = int(rng.uniform(0.0, 0.1) * V)
+        traps = rng.choice(V, nt, replace=False)
+        depth = np.clip(rng.normal(0.3, 0.1, nt), 0.1, 0.5)
+        holes = rng.choice(V, n, replace=False).astype(np.int32)
+        reps.append(dict(params=p, holes=holes, potential=background_potential(p, traps, depth)))
+    return reps
+
+
+def bench_batch(steps=200, warmup=20, n_rep=256):
+    import langmuir_amd as L
+    reps = batch_workload(n_rep)
+    t0 = time.perf_counter()
+    batch = L.Batch(reps)
+    log(f"batch of {n_rep} replicas created in {time.perf_counter() - t0:.1f} s")
+    batch.step(warmup)
+    batch.sync()
+    ns = [len(batch.state(r)["h_site"]) for r in range(n_rep)]
+    t0 = time.perf_counter()
+    ms = batch.step_timed(steps)
+    wall = time.perf_counter() - t0
+    pair_evals = sum(2.0 * n * n for n in ns)
+    out = dict(replicas=n_rep, steps=steps, ms_per_step=round(wall * 1e3 / steps, 4),
+               event_ms_per_step=round(ms / steps, 4),
+               replica_steps_per_s=round(n_rep * steps / wall, 1),
+               pair_evals_per_s=pair_evals * steps / wall,
+               carriers_total=int(sum(ns)))
+    log(f"batch: {out['ms_per_step']} ms/step for {n_rep} replicas -> "
+        f"{out['replica_steps_per_s']:.0f} replica-steps/s")
+    batch.close()
+    return out
+
+
 def cpu_baseline(workload, budget_s=15.0):
     """FP64 oracle (restated reference CPU path) on the same workload, all host cores."""
     from oracle import pyoracle as O
@@ -222,6 +274,7 @@ def main():
     ap.add_argument("--workload", choices=("solar", "transistor"), default="solar")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-map", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -295,6 +348,8 @@ def main():
     }
     if not args.no_map:
         res["map"] = bench_map()
+    if not args.no_batch:
+        res["batch"] = bench_batch()
     res["surrogate"] = None
     if not args.no_cpu_baseline:
         cb = cpu_baseline(workload, args.cpu_budget)

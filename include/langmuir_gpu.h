@@ -175,6 +175,27 @@ lmg_status lmg_sim_delta_e(lmg_ctx* ctx, int32_t* fsite_e, int32_t* fsite_h,
  * reference's qFatal "can not balance"). */
 lmg_status lmg_sim_device_error(lmg_ctx* ctx, int32_t* code);
 
+/* ---------------------------------------------------------------------------
+ * Batched replicas: n independent simulations (each with its own lattice,
+ * cutoff, carriers, traps and seed) advanced together, one launch per kernel
+ * per step for all of them.  The reference ran independent jobs as separate
+ * processes (LangmuirPython/batch/frank/gpu.batch:1-18, LangmuirPython/utils/
+ * scan.py:13); there is no in-process counterpart.  Replicas must agree on
+ * coulomb_carriers and on Gaussian screening (sigma > 0 or not).
+ * ------------------------------------------------------------------------- */
+typedef struct lmg_batch lmg_batch;
+
+lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
+                            const lmg_sim_state* states, int32_t device_id, lmg_batch** out);
+void lmg_batch_destroy(lmg_batch* batch);
+const char* lmg_batch_last_error(const lmg_batch* batch);
+int32_t lmg_batch_size(const lmg_batch* batch);
+lmg_status lmg_batch_step(lmg_batch* batch, int32_t n_steps);
+/* n steps timed with HIP events on the batch stream; *ms = event span */
+lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
+lmg_status lmg_batch_sync(lmg_batch* batch);
+lmg_status lmg_batch_store(lmg_batch* batch, int32_t replica, lmg_sim_state* state);
+
 #ifdef __cplusplus
 }
 #endif

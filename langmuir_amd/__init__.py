@@ -4,13 +4,13 @@ The device work lives in ``liblangmuir_gpu.so`` (HIP for gfx950, C-ABI in
 ``include/langmuir_gpu.h``); this package is the host-side mirror of the
 reference's interfaces over that C-ABI (see :mod:`langmuir_amd.host`).
 """
-from .host import (Context, GpuHelper, Simulation, SimulationParameters, background_potential,
+from .host import (Batch, Context, GpuHelper, Simulation, SimulationParameters, background_potential,
                    coupling_table, device_count, drain_slots, flux_rates, make_config,
                    mt19937_seed_window)
 from ._abi import LMG_GAUSS, LMG_IMAGE_LEFT, LMG_NO_CUTOFF, LmgError
 
 __all__ = [
-    "Context", "GpuHelper", "Simulation", "SimulationParameters", "background_potential",
+    "Batch", "Context", "GpuHelper", "Simulation", "SimulationParameters", "background_potential",
     "coupling_table", "device_count", "drain_slots", "flux_rates", "make_config",
     "mt19937_seed_window", "LMG_GAUSS", "LMG_IMAGE_LEFT", "LMG_NO_CUTOFF", "LmgError",
 ]

@@ -105,6 +105,17 @@ def lib():
     L.lmg_sim_delta_e.argtypes = [VP, VP, VP, VP, VP]
     L.lmg_sim_device_error.argtypes = [VP, PI32]
     L.lmg_sim_step_timed.argtypes = [VP, I32, PF64]
+    L.lmg_batch_create.argtypes = [I32, VP, VP, VP, I32, C.POINTER(VP)]
+    L.lmg_batch_destroy.argtypes = [VP]
+    L.lmg_batch_destroy.restype = None
+    L.lmg_batch_last_error.argtypes = [VP]
+    L.lmg_batch_last_error.restype = C.c_char_p
+    L.lmg_batch_size.argtypes = [VP]
+    L.lmg_batch_size.restype = I32
+    L.lmg_batch_step.argtypes = [VP, I32]
+    L.lmg_batch_step_timed.argtypes = [VP, I32, PF64]
+    L.lmg_batch_sync.argtypes = [VP]
+    L.lmg_batch_store.argtypes = [VP, I32, C.POINTER(LmgSimState)]
     _lib = L
     return L
 
@@ -114,4 +125,6 @@ EXPORTED = (
     "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_potential_map", "lmg_potential_map_dev",
     "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_delta_e",
     "lmg_sim_device_error", "lmg_sim_step_timed",
+    "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
+    "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_sync", "lmg_batch_store",
 )

@@ -811,3 +811,194 @@ lmg_status lmg_sim_delta_e(lmg_ctx* c, int32_t* fsite_e, int32_t* fsite_h, doubl
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched replicas: many independent simulations, each kernel of a step
+// launched once for all of them (blockIdx.z = replica).
+// ===========================================================================
+struct lmg_batch {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<lmg_ctx*> reps;
+    SimDev* d_sims = nullptr;
+    int max_nblk = 0, max_S = 0;
+    bool coulomb = false, gauss = false, any_uniform = false, any_recomb = false;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+lmg_status bfail(lmg_batch* b, lmg_status code, const std::string& msg) {
+    if (b) b->err = msg;
+    else g_batch_error = msg;
+    return code;
+}
+
+#define BHIPCHK(b, call)                                                               \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return bfail((b), LMG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+lmg_status batch_launch_step(lmg_batch* b) {
+    const dim3 blk(kSimBlock);
+    const int R = int(b->reps.size());
+    const dim3 g_de(b->max_nblk + 1, b->coulomb ? b->max_S : 1, R);
+    const dim3 g_grid(b->max_nblk, 1, R), g_one(1, 1, R);
+    if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_sims);
+    else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_sims);
+    else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_sims);
+    hipLaunchKernelGGL(decide_kernel, g_grid, blk, 0, b->stream, b->d_sims, 0);
+    if (b->any_recomb) hipLaunchKernelGGL(recomb_kernel, g_one, dim3(kOneBlock), 0, b->stream, b->d_sims);
+    hipLaunchKernelGGL(tick_kernel, g_grid, blk, 0, b->stream, b->d_sims);
+    hipLaunchKernelGGL(inject_kernel, g_one, dim3(kOneBlock), 0, b->stream, b->d_sims, 0);
+    if (b->any_uniform) {
+        hipLaunchKernelGGL(propose_kernel, g_grid, blk, 0, b->stream, b->d_sims);
+        hipLaunchKernelGGL(propose_fix_kernel, g_one, dim3(64), 0, b->stream, b->d_sims);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, LMG_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
+    return LMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lmg_batch_last_error(const lmg_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+void lmg_batch_destroy(lmg_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (lmg_ctx* c : b->reps) lmg_destroy(c);
+    if (b->d_sims) (void)hipFree(b->d_sims);
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
+                            const lmg_sim_state* states, int32_t device_id, lmg_batch** out) {
+    if (!out || n < 1 || !params || !configs || !states) return bfail(nullptr, LMG_ERR_ARG, "invalid batch arguments");
+    *out = nullptr;
+    lmg_batch* b = new lmg_batch;
+    b->dev = device_id;
+    long long total_waves = 0;
+    for (int r = 0; r < n; ++r) {
+        lmg_params p = params[r];
+        p.device_id = device_id;
+        lmg_ctx* c = nullptr;
+        lmg_status st = lmg_create(&p, &c);
+        if (st != LMG_OK) {
+            bfail(nullptr, st, "replica " + std::to_string(r) + ": " + lmg_last_error(nullptr));
+            lmg_batch_destroy(b);
+            return st;
+        }
+        b->reps.push_back(c);
+        st = lmg_sim_load(c, &configs[r], &states[r]);
+        if (st != LMG_OK) {
+            bfail(nullptr, st, "replica " + std::to_string(r) + ": " + lmg_last_error(c));
+            lmg_batch_destroy(b);
+            return st;
+        }
+        const SimDev& D = c->sim;
+        if (r == 0) {
+            b->coulomb = D.coulomb;
+            b->gauss = D.gauss;
+        } else if (b->coulomb != bool(D.coulomb) || (D.coulomb && b->gauss != bool(D.gauss))) {
+            bfail(nullptr, LMG_ERR_ARG, "replicas must agree on coulomb.carriers and Gaussian screening");
+            lmg_batch_destroy(b);
+            return LMG_ERR_ARG;
+        }
+        total_waves += (D.cap_e + D.cap_h + 63) / 64;
+    }
+    if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        bfail(nullptr, LMG_ERR_HIP, "batch stream creation failed");
+        lmg_batch_destroy(b);
+        return LMG_ERR_HIP;
+    }
+    // the replicas already fill the GPU: split each replica's charges into fewer chunks
+    const int S_common = int(std::max<long long>(1, std::min<long long>(64, (8192 + total_waves - 1) / total_waves)));
+    std::vector<SimDev> host(n);
+    for (int r = 0; r < n; ++r) {
+        SimDev& D = b->reps[r]->sim;
+        D.S = std::max(1, std::min(D.S, S_common));
+        host[r] = D;
+        b->max_nblk = std::max(b->max_nblk, D.nblk);
+        b->max_S = std::max(b->max_S, D.S);
+        b->any_uniform |= D.uniform != 0;
+        b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
+        // the single-context descriptor follows the same chunking
+        if (hipMemcpy(b->reps[r]->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice) != hipSuccess) {
+            bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
+            lmg_batch_destroy(b);
+            return LMG_ERR_HIP;
+        }
+    }
+    if (hipMalloc(&b->d_sims, sizeof(SimDev) * n) != hipSuccess ||
+        hipMemcpy(b->d_sims, host.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        bfail(nullptr, LMG_ERR_HIP, "descriptor array allocation failed");
+        lmg_batch_destroy(b);
+        return LMG_ERR_HIP;
+    }
+    *out = b;
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_step(lmg_batch* b, int32_t n) {
+    if (!b) return LMG_ERR_ARG;
+    if (n < 0) return bfail(b, LMG_ERR_ARG, "negative step count");
+    BHIPCHK(b, hipSetDevice(b->dev));
+    for (int i = 0; i < n; ++i) {
+        lmg_status st = batch_launch_step(b);
+        if (st != LMG_OK) return st;
+    }
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_step_timed(lmg_batch* b, int32_t n, double* kernel_ms) {
+    if (!b || !kernel_ms || n < 1) return LMG_ERR_ARG;
+    BHIPCHK(b, hipSetDevice(b->dev));
+    while (b->events.size() < 2) {
+        hipEvent_t e;
+        BHIPCHK(b, hipEventCreate(&e));
+        b->events.push_back(e);
+    }
+    BHIPCHK(b, hipEventRecord(b->events[0], b->stream));
+    for (int i = 0; i < n; ++i) {
+        lmg_status st = batch_launch_step(b);
+        if (st != LMG_OK) return st;
+    }
+    BHIPCHK(b, hipEventRecord(b->events[1], b->stream));
+    BHIPCHK(b, hipEventSynchronize(b->events[1]));
+    float ms = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->events[0], b->events[1]));
+    kernel_ms[0] = ms;
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_sync(lmg_batch* b) {
+    if (!b) return LMG_ERR_ARG;
+    BHIPCHK(b, hipSetDevice(b->dev));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_store(lmg_batch* b, int32_t replica, lmg_sim_state* state) {
+    if (!b || !state || replica < 0 || replica >= int(b->reps.size()))
+        return bfail(b, LMG_ERR_ARG, "invalid replica index");
+    BHIPCHK(b, hipSetDevice(b->dev));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    lmg_status st = lmg_sim_store(b->reps[replica], state);
+    if (st != LMG_OK) return bfail(b, st, lmg_last_error(b->reps[replica]));
+    return LMG_OK;
+}
+
+int32_t lmg_batch_size(const lmg_batch* b) { return b ? int32_t(b->reps.size()) : 0; }
+
+}  // extern "C"

@@ -177,10 +177,11 @@ __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsi
 }
 
 // ===========================================================================
-// de_kernel: grid = (nblk + 1, S).  Workgroup (x, s) sums charge chunk s for
-// carriers [256x, 256x + 256); chunk-0 workgroups also flag the carriers whose
-// decision draws (own-grid type Empty or Drain at the proposed site,
-// chargeagent.cpp:95-157).  Workgroup (nblk, 0) refills the random stream.
+// de_kernel: grid = (nblk + 1, S, replicas) (the largest replica's extents;
+// smaller replicas exit surplus workgroups).  Workgroup (x, s) sums charge
+// chunk s for carriers [256x, 256x + 256); chunk-0 workgroups also flag the
+// carriers whose decision draws (own-grid type Empty or Drain at the proposed
+// site, chargeagent.cpp:95-157).  Workgroup (nblk, 0) refills the random stream.
 //
 // For each charge j it accumulates q_j ( [0<r2<c2] f(r2) - [0<r1<c2] f(r1) ),
 // the pair part of p2 - p1 in ChargeAgent::coulombCPU (chargeagent.cpp:215-271).
@@ -212,11 +213,12 @@ __device__ __forceinline__ float de_term(const float4& p1, const float4& q, floa
 
 template <bool kGauss, bool kPairs>
 __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];     // replica
     __shared__ float4 tile[256];
     __shared__ uint32_t tail[1024];
-    if (int(blockIdx.x) == D.nblk) {
-        if (blockIdx.y == 0) mt_refill(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
+    if (int(blockIdx.x) >= D.nblk || int(blockIdx.y) >= D.S) {
+        if (int(blockIdx.x) == D.nblk && blockIdx.y == 0)
+            mt_refill(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     const int ne = D.ctl->n_e, nt = D.ctl->counts[0], nc = D.ctl->counts[1];
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
         if (threadIdx.x == 0) D.blk_draw[blockIdx.x] = cnt;
     }
     if (!kPairs) return;
-    const int S = gridDim.y;
+    const int S = D.S;
     const int chunk = (nc + S - 1) / S;
     const int j0 = blockIdx.y * chunk, j1 = min(nc, j0 + chunk);
     const float4 p1 = i < nt ? D.tgt1[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 // only_de = 1 assembles dE (chargeagent.cpp:215-296) and stops.
 // ===========================================================================
 __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restrict__ Dp, int only_de) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
     __shared__ int wsum[4];
     __shared__ long long pre;
     __shared__ unsigned int att[4], suc[4];
@@ -442,7 +444,8 @@ __device__ double wg_sum_d(double v, double* red) {
 // other->removed(), so thread 0 walks the list.  Re-counts the kept carriers.
 // ===========================================================================
 __global__ __launch_bounds__(kOneBlock) void recomb_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
+    if (!(D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0)) return;
     __shared__ OneShared sh;
     SimCtl& C = *D.ctl;
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(kOneBlock) void recomb_kernel(const SimDev* __restr
 // recombined carriers leave; survivors are compacted in order.
 // ===========================================================================
 __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
     __shared__ int wsum[4];
     __shared__ long long pre_e, pre_h;
     const SimCtl& C = *D.ctl;
@@ -794,7 +797,7 @@ __device__ unsigned long long propose_serial(const SimDev& D, int i0, int ne, in
 // Leaves the new lists' sizes, the proposal start rng_step0 and counts in ctl;
 // with D.uniform == 0 it also makes the proposals (serially).
 __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restrict__ Dp, int mode) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
     __shared__ OneShared sh;
     SimCtl& C = *D.ctl;
     if (threadIdx.x == 0) {
@@ -840,7 +843,8 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
 // on both x faces): carrier i uses engine output rng_step0 + i unless a Boost
 // rejection happened before it; the first rejecting index is recorded.
 __global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
+    if (!D.uniform) return;             // inject_kernel proposed serially
     const SimCtl& C = *D.ctl;
     const int ne = C.n_e, N = ne + C.n_h, cur = C.cur;
     const int i = blockIdx.x * kSimBlock + threadIdx.x;
@@ -856,9 +860,9 @@ __global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __rest
 // After propose_kernel: redo serially from the first rejection (if any) and
 // set the engine position of the pending decisions.
 __global__ void propose_fix_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+    const SimDev& D = Dp[blockIdx.z];   // replica
     SimCtl& C = *D.ctl;
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0 || !D.uniform) return;
     const int ne = C.n_e, N = ne + C.n_h;
     const unsigned long long base = C.rng_step0;
     if (base + N + 624ull > C.gen_pos) set_error(D, kErrRing);

@@ -419,6 +419,135 @@ def make_config(p: SimulationParameters) -> LmgSimConfig:
     return cfg
 
 
+def pack_state(p: SimulationParameters, electrons=(), holes=(), defects=(), potential=None,
+               rng_window=None, rng_consumed=0, e_lifetime=None, e_pathlength=None, h_lifetime=None,
+               h_pathlength=None, flux_attempts=None, flux_successes=None, current_step=0):
+    """An lmg_sim_state for one configuration; returns (state, arrays it points into)."""
+    e, h, d = _i32(list(electrons)), _i32(list(holes)), _i32(list(defects))
+    pot = np.ascontiguousarray(background_potential(p) if potential is None else potential,
+                               dtype=np.float64)
+    if len(pot) != p.volume:
+        raise ValueError("potential must have V entries")
+    win = mt19937_seed_window(p.random_seed) if rng_window is None else np.asarray(rng_window, np.uint32)
+    arrays = dict(e=e, h=h, d=d, pot=pot,
+                  el=_i32(np.zeros(len(e)) if e_lifetime is None else e_lifetime),
+                  ep=_i32(np.zeros(len(e)) if e_pathlength is None else e_pathlength),
+                  hl=_i32(np.zeros(len(h)) if h_lifetime is None else h_lifetime),
+                  hp=_i32(np.zeros(len(h)) if h_pathlength is None else h_pathlength))
+    st = LmgSimState()
+    st.n_electrons, st.n_holes, st.n_defects = len(e), len(h), len(d)
+    pi = lambda a: a.ctypes.data_as(_abi.PI32)
+    st.e_site, st.e_lifetime, st.e_pathlength = pi(arrays["e"]), pi(arrays["el"]), pi(arrays["ep"])
+    st.h_site, st.h_lifetime, st.h_pathlength = pi(arrays["h"]), pi(arrays["hl"]), pi(arrays["hp"])
+    st.defect_site = pi(arrays["d"])
+    st.potential = arrays["pot"].ctypes.data_as(_abi.PF64)
+    for i in range(624):
+        st.rng_window[i] = int(win[i])
+    st.rng_consumed = int(rng_consumed)
+    fa = np.zeros(10, np.uint64) if flux_attempts is None else np.asarray(flux_attempts, np.uint64)
+    fs = np.zeros(10, np.uint64) if flux_successes is None else np.asarray(flux_successes, np.uint64)
+    for i in range(10):
+        st.flux_attempts[i] = int(fa[i])
+        st.flux_successes[i] = int(fs[i])
+    st.current_step = int(current_step)
+    return st, arrays
+
+
+def lmg_params(p: SimulationParameters, device_id: int = 0) -> LmgParams:
+    return LmgParams(grid_x=p.grid_x, grid_y=p.grid_y, grid_z=p.grid_z, cutoff=p.electrostatic_cutoff,
+                     hopping_range=p.hopping_range, defects_charge=p.defects_charge,
+                     device_id=device_id, reserved=0, prefactor=p.prefactor, inverse_kt=p.inverse_kt,
+                     gaussian_sigma=p.coulomb_gaussian_sigma, exciton_binding=p.exciton_binding)
+
+
+def _unpack_state(st: LmgSimState, bufs) -> dict:
+    ne, nh = st.n_electrons, st.n_holes
+    return dict(
+        e_site=bufs["es"][:ne].copy(), e_lifetime=bufs["el"][:ne].copy(),
+        e_pathlength=bufs["ep"][:ne].copy(), h_site=bufs["hs"][:nh].copy(),
+        h_lifetime=bufs["hl"][:nh].copy(), h_pathlength=bufs["hp"][:nh].copy(),
+        rng_window=np.array(st.rng_window[:], dtype=np.uint32), rng_consumed=int(st.rng_consumed),
+        flux_attempts=np.array(st.flux_attempts[:], dtype=np.uint64),
+        flux_successes=np.array(st.flux_successes[:], dtype=np.uint64),
+        current_step=int(st.current_step))
+
+
+def _store_buffers(cap_e: int, cap_h: int):
+    bufs = {k: np.zeros(cap_e if k[0] == "e" else cap_h, np.int32)
+            for k in ("es", "el", "ep", "hs", "hl", "hp")}
+    st = LmgSimState()
+    pi = lambda a: a.ctypes.data_as(_abi.PI32)
+    st.e_site, st.e_lifetime, st.e_pathlength = pi(bufs["es"]), pi(bufs["el"]), pi(bufs["ep"])
+    st.h_site, st.h_lifetime, st.h_pathlength = pi(bufs["hs"]), pi(bufs["hl"]), pi(bufs["hp"])
+    return st, bufs
+
+
+class Batch:
+    """Many independent simulations advanced together (lmg_batch_*).
+
+    ``replicas`` is a list of dicts with a ``params`` (SimulationParameters) entry and
+    the keyword arguments of :func:`pack_state` (electrons, holes, potential, ...).
+    """
+
+    def __init__(self, replicas, device_id: int = 0):
+        L = _abi.lib()
+        n = len(replicas)
+        self._params = (LmgParams * n)()
+        self._configs = (LmgSimConfig * n)()
+        self._states = (LmgSimState * n)()
+        self._keep = []
+        self.caps = []
+        for r, rep in enumerate(replicas):
+            rep = dict(rep)
+            p = rep.pop("params")
+            p.check()
+            self._params[r] = lmg_params(p, device_id)
+            self._configs[r] = make_config(p)
+            st, arrays = pack_state(p, **rep)
+            self._states[r] = st
+            self._keep.append(arrays)
+            self.caps.append((max(self._configs[r].max_electrons, st.n_electrons, 1),
+                              max(self._configs[r].max_holes, st.n_holes, 1)))
+        h = C.c_void_p()
+        rc = L.lmg_batch_create(n, C.cast(self._params, C.c_void_p), C.cast(self._configs, C.c_void_p),
+                                C.cast(self._states, C.c_void_p), device_id, C.byref(h))
+        if rc != 0:
+            raise LmgError(rc, L.lmg_batch_last_error(None).decode())
+        self._h = h
+
+    def __len__(self):
+        return int(_abi.lib().lmg_batch_size(self._h))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise LmgError(rc, _abi.lib().lmg_batch_last_error(self._h).decode())
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _abi.lib().lmg_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n: int):
+        self._chk(_abi.lib().lmg_batch_step(self._h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = C.c_double()
+        self._chk(_abi.lib().lmg_batch_step_timed(self._h, int(n), C.byref(ms)))
+        return float(ms.value)
+
+    def sync(self):
+        self._chk(_abi.lib().lmg_batch_sync(self._h))
+
+    def state(self, replica: int) -> dict:
+        st, bufs = _store_buffers(*self.caps[replica])
+        self._chk(_abi.lib().lmg_batch_store(self._h, int(replica), C.byref(st)))
+        return _unpack_state(st, bufs)
+
+
 class Simulation:
     """Simulation::performIterations on the device for one configuration.
 
@@ -447,36 +576,11 @@ class Simulation:
     def load(self, electrons, holes, defects=(), potential=None, rng_window=None, rng_consumed=0,
              e_lifetime=None, e_pathlength=None, h_lifetime=None, h_pathlength=None,
              flux_attempts=None, flux_successes=None, current_step=0):
-        p = self.params
-        e, h, d = _i32(list(electrons)), _i32(list(holes)), _i32(list(defects))
-        self.cap_e = max(self.cap_e, len(e))
-        self.cap_h = max(self.cap_h, len(h))
-        pot = np.ascontiguousarray(background_potential(p) if potential is None else potential,
-                                   dtype=np.float64)
-        if len(pot) != p.volume:
-            raise ValueError("potential must have V entries")
-        win = mt19937_seed_window(p.random_seed) if rng_window is None else np.asarray(rng_window, np.uint32)
-        arrays = dict(e=e, h=h, d=d, pot=pot,
-                      el=_i32(np.zeros(len(e)) if e_lifetime is None else e_lifetime),
-                      ep=_i32(np.zeros(len(e)) if e_pathlength is None else e_pathlength),
-                      hl=_i32(np.zeros(len(h)) if h_lifetime is None else h_lifetime),
-                      hp=_i32(np.zeros(len(h)) if h_pathlength is None else h_pathlength))
-        st = LmgSimState()
-        st.n_electrons, st.n_holes, st.n_defects = len(e), len(h), len(d)
-        pi = lambda a: a.ctypes.data_as(_abi.PI32)
-        st.e_site, st.e_lifetime, st.e_pathlength = pi(arrays["e"]), pi(arrays["el"]), pi(arrays["ep"])
-        st.h_site, st.h_lifetime, st.h_pathlength = pi(arrays["h"]), pi(arrays["hl"]), pi(arrays["hp"])
-        st.defect_site = pi(arrays["d"])
-        st.potential = arrays["pot"].ctypes.data_as(_abi.PF64)
-        for i in range(624):
-            st.rng_window[i] = int(win[i])
-        st.rng_consumed = int(rng_consumed)
-        fa = np.zeros(10, np.uint64) if flux_attempts is None else np.asarray(flux_attempts, np.uint64)
-        fs = np.zeros(10, np.uint64) if flux_successes is None else np.asarray(flux_successes, np.uint64)
-        for i in range(10):
-            st.flux_attempts[i] = int(fa[i])
-            st.flux_successes[i] = int(fs[i])
-        st.current_step = int(current_step)
+        st, keep = pack_state(self.params, electrons, holes, defects, potential, rng_window,
+                              rng_consumed, e_lifetime, e_pathlength, h_lifetime, h_pathlength,
+                              flux_attempts, flux_successes, current_step)
+        self.cap_e = max(self.cap_e, st.n_electrons)
+        self.cap_h = max(self.cap_h, st.n_holes)
         self._chk(_abi.lib().lmg_sim_load(self.ctx.handle, C.byref(self.config), C.byref(st)))
 
     def _chk(self, rc):
@@ -504,22 +608,9 @@ class Simulation:
         return int(code.value)
 
     def state(self) -> dict:
-        st = LmgSimState()
-        bufs = {k: np.zeros(self.cap_e if k[0] == "e" else self.cap_h, np.int32)
-                for k in ("es", "el", "ep", "hs", "hl", "hp")}
-        pi = lambda a: a.ctypes.data_as(_abi.PI32)
-        st.e_site, st.e_lifetime, st.e_pathlength = pi(bufs["es"]), pi(bufs["el"]), pi(bufs["ep"])
-        st.h_site, st.h_lifetime, st.h_pathlength = pi(bufs["hs"]), pi(bufs["hl"]), pi(bufs["hp"])
+        st, bufs = _store_buffers(self.cap_e, self.cap_h)
         self._chk(_abi.lib().lmg_sim_store(self.ctx.handle, C.byref(st)))
-        ne, nh = st.n_electrons, st.n_holes
-        return dict(
-            e_site=bufs["es"][:ne].copy(), e_lifetime=bufs["el"][:ne].copy(),
-            e_pathlength=bufs["ep"][:ne].copy(), h_site=bufs["hs"][:nh].copy(),
-            h_lifetime=bufs["hl"][:nh].copy(), h_pathlength=bufs["hp"][:nh].copy(),
-            rng_window=np.array(st.rng_window[:], dtype=np.uint32), rng_consumed=int(st.rng_consumed),
-            flux_attempts=np.array(st.flux_attempts[:], dtype=np.uint64),
-            flux_successes=np.array(st.flux_successes[:], dtype=np.uint64),
-            current_step=int(st.current_step))
+        return _unpack_state(st, bufs)
 
     def delta_e(self):
         """(fsite_e, fsite_h, dE_e, dE_h) of the pending proposals."""

@@ -1,0 +1,68 @@
+"""Batched replicas (lmg_batch_*): each replica of a heterogeneous batch (own
+lattice, cutoff, carrier count, traps and seed) must follow exactly the trajectory
+it follows alone, and the oracle's."""
+import numpy as np
+import pytest
+
+from _util import compare_state, de_tolerance, oracle_state, params_pair
+
+pytestmark = pytest.mark.gpu
+
+REPLICAS = [
+    dict(grid_x=24, grid_y=20, grid_z=1, hole_percentage=0.05, electrostatic_cutoff=20, random_seed=31),
+    dict(grid_x=16, grid_y=16, grid_z=8, hole_percentage=0.02, electrostatic_cutoff=12, random_seed=32),
+    dict(grid_x=40, grid_y=12, grid_z=3, hole_percentage=0.04, electrostatic_cutoff=50, random_seed=33),
+    dict(simulation_type=1, grid_x=12, grid_y=12, grid_z=12, electron_percentage=0.02,
+         hole_percentage=0.02, seed_charges=0.5, generation_rate=0.4, electrostatic_cutoff=9,
+         recombination_rate=0.2, recombination_range=1, random_seed=34),
+    dict(grid_x=32, grid_y=32, grid_z=1, hole_percentage=0.08, electrostatic_cutoff=30, random_seed=35),
+    dict(grid_x=8, grid_y=8, grid_z=8, hole_percentage=0.05, electrostatic_cutoff=6, random_seed=36),
+]
+BASE = dict(coulomb_carriers=1, electron_percentage=0.0, seed_charges=1.0, voltage_left=1.0,
+            e_source_l_rate=0.0, h_source_l_rate=0.9, h_drain_r_rate=0.9)
+
+
+def build(oracle, gpu):
+    worlds, reps = [], []
+    for kw in REPLICAS:
+        k = dict(BASE)
+        if kw.get("simulation_type") == 1:
+            k = dict(coulomb_carriers=1)
+        k.update(kw)
+        op, sp = params_pair(**k)
+        w = oracle.World(op)
+        st = oracle_state(w)
+        worlds.append((w, sp, st))
+        reps.append(dict(params=sp, **st))
+    return worlds, gpu.Batch(reps)
+
+
+def test_batch_matches_single_contexts_and_oracle(gpu, oracle):
+    worlds, batch = build(oracle, gpu)
+    assert len(batch) == len(REPLICAS)
+    singles = [gpu.Simulation(sp, **st) for (_, sp, st) in worlds]
+    for t in range(40):
+        batch.step(1)
+        for s in singles:
+            s.step(1)
+        for r, (w, sp, _) in enumerate(worlds):
+            w.step(1)
+            bs = batch.state(r)
+            ss = singles[r].state()
+            for k in ("e_site", "h_site", "e_pathlength", "h_pathlength", "rng_window",
+                      "flux_attempts", "flux_successes"):
+                assert np.array_equal(bs[k], ss[k]), f"replica {r} step {t}: {k} differs from its single run"
+            assert bs["rng_consumed"] == ss["rng_consumed"]
+            assert not compare_state(bs, w), f"replica {r} step {t}: differs from the oracle"
+
+
+def test_batch_determinism_and_invariants(gpu, oracle):
+    _, a = build(oracle, gpu)
+    _, b = build(oracle, gpu)
+    a.step(60)
+    b.step(60)
+    for r in range(len(REPLICAS)):
+        sa, sb = a.state(r), b.state(r)
+        assert np.array_equal(sa["h_site"], sb["h_site"]) and np.array_equal(sa["e_site"], sb["e_site"])
+        assert len(np.unique(sa["h_site"])) == len(sa["h_site"])
+        assert sa["current_step"] == 60
