@@ -72,14 +72,17 @@ int32_t lmg_device_count(void);
 lmg_status lmg_sync(lmg_ctx* ctx);
 /* The context's HIP stream (hipStream_t), for callers that order their own work. */
 void* lmg_stream(lmg_ctx* ctx);
+/* Copy of the parameters the context was created with (the World fields
+ * OpenClHelper reads back through m_world, openclhelper.cpp:124-168). */
+lmg_status lmg_get_params(const lmg_ctx* ctx, lmg_params* out);
 
 /* ---------------------------------------------------------------------------
  * Pair sums at an explicit target list: the coulomb2/gauss2 kernels with
  * launchCoulombKernel2/launchGaussKernel2 + getOutputHost
  * (kernel.cl:222-369, openclhelper.cpp:372-544,567-583), and the image sums
  * Potential::coulombImageE/H/D.  out[t] = pref * sum_j q_j f(r_tj) over
- * 0 < r^2 < cutoff^2 (times -1 for LMG_IMAGE_LEFT), FP32 terms with
- * compensated accumulation, delivered as double.
+ * 0 < r^2 < cutoff^2 (times -1 for LMG_IMAGE_LEFT; every r > 0 with
+ * LMG_NO_CUTOFF), FP32 terms with compensated accumulation, delivered as double.
  * ------------------------------------------------------------------------- */
 lmg_status lmg_pair_potential(lmg_ctx* ctx, const int32_t* charge_site,
                               const int32_t* charge_q, int32_t n_charges,
@@ -195,6 +198,62 @@ lmg_status lmg_batch_step(lmg_batch* batch, int32_t n_steps);
 lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
 lmg_status lmg_batch_sync(lmg_batch* batch);
 lmg_status lmg_batch_store(lmg_batch* batch, int32_t replica, lmg_sim_state* state);
+
+/* ---------------------------------------------------------------------------
+ * Learned surrogate of the potential map.  No counterpart in the reference
+ * (its only map is the exact coulomb1/gauss1 kernel, kernel.cl:56-220): a
+ * spectral operator map = window(irfft3(W . rfft3(pad(q)))) with complex
+ * weights W on the half spectrum of a zero-padded grid (padded[] per axis,
+ * >= N + cutoff - 1, or >= 2N - 1 with LMG_NO_CUTOFF), trained on the GPU from
+ * (occupancy, exact map) pairs.  It approximates lmg_potential_map_dev with the
+ * flags given at creation (LMG_GAUSS, LMG_NO_CUTOFF) for the context's lattice
+ * and uses the context's stream.
+ *
+ * Occupancy grids are int8 charge sums per site (V bytes per grid, grids
+ * contiguous); a single grid written by lmg_surrogate_occupancy_dev must start
+ * 4-byte aligned and hold ceil(V / 4) * 4 bytes.  Weights are n_weights
+ * interleaved (re, im) float pairs in [kz][ky][kx] order, kx < padded[0]/2 + 1.
+ * ------------------------------------------------------------------------- */
+typedef struct lmg_surrogate lmg_surrogate;
+
+/* max_batch bounds the batch of infer/gradient/train calls (device memory is
+ * sized for it).  Weights start at zero. */
+lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, lmg_surrogate** out);
+void lmg_surrogate_destroy(lmg_surrogate* s);
+const char* lmg_surrogate_last_error(const lmg_surrogate* s);
+lmg_status lmg_surrogate_shape(const lmg_surrogate* s, int32_t* padded_xyz, int64_t* n_weights);
+lmg_status lmg_surrogate_get_weights(lmg_surrogate* s, float* w);
+/* w == NULL zeroes the weights; either way the momentum is reset. */
+lmg_status lmg_surrogate_set_weights(lmg_surrogate* s, const float* w);
+/* int8 occupancy grid from a charge list (sites < V, |q| <= 127; a bad entry is
+ * reported by the next lmg_surrogate_map_dev or device sync). */
+lmg_status lmg_surrogate_occupancy_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n,
+                                       int8_t* d_occ);
+/* batch maps (float[V] each) from batch occupancy grids */
+lmg_status lmg_surrogate_infer_dev(lmg_surrogate* s, const int8_t* d_occ, int32_t batch, float* d_map);
+/* Gradient of L = sum_b |map_b - target_b|^2 / sum_b |target_b|^2 with respect
+ * to the weights (2 n_weights floats, device) at the current weights; *loss =
+ * L.  Synchronous. */
+lmg_status lmg_surrogate_gradient_dev(lmg_surrogate* s, const int8_t* d_occ, const float* d_target, int32_t batch,
+                                      float* d_grad, double* loss);
+/* iterations steps of momentum SGD on random minibatches of `batch` of the
+ * n_snapshots (occupancy, target) pairs, starting from the current weights.
+ * Each step adds a real update that is even in ky and kz (the symmetry of the
+ * exact operator under lattice reflections), preconditioned per frequency by
+ * the inverse mean spectral power of the first min(n, 256) occupancies; the
+ * step size is 0.9 (1 - momentum) / lambda_max with lambda_max from a power
+ * iteration on the preconditioned Hessian.  loss_history (host, may be NULL)
+ * receives each step's minibatch loss.  Synchronous. */
+lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const float* d_target, int32_t n_snapshots,
+                                   int32_t batch, int32_t iterations, double momentum, uint64_t seed,
+                                   float* loss_history);
+/* Gated map of a charge list: surrogate map, exact pair sums at gate_sites
+ * sampled sites (seeded), *gate_error = relative RMS error over them; when it
+ * exceeds tolerance the exact map (lmg_potential_map_dev) overwrites d_map and
+ * *used_exact = 1.  Synchronous (the gate decision is taken on the host). */
+lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n, float* d_map,
+                                 double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
+                                 int32_t* used_exact);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,20 @@ def lib():
     L.lmg_batch_step_timed.argtypes = [VP, I32, PF64]
     L.lmg_batch_sync.argtypes = [VP]
     L.lmg_batch_store.argtypes = [VP, I32, C.POINTER(LmgSimState)]
+    L.lmg_get_params.argtypes = [VP, C.POINTER(LmgParams)]
+    L.lmg_surrogate_create.argtypes = [VP, I32, I32, C.POINTER(VP)]
+    L.lmg_surrogate_destroy.argtypes = [VP]
+    L.lmg_surrogate_destroy.restype = None
+    L.lmg_surrogate_last_error.argtypes = [VP]
+    L.lmg_surrogate_last_error.restype = C.c_char_p
+    L.lmg_surrogate_shape.argtypes = [VP, PI32, C.POINTER(I64)]
+    L.lmg_surrogate_get_weights.argtypes = [VP, VP]
+    L.lmg_surrogate_set_weights.argtypes = [VP, VP]
+    L.lmg_surrogate_occupancy_dev.argtypes = [VP, VP, VP, I32, VP]
+    L.lmg_surrogate_infer_dev.argtypes = [VP, VP, I32, VP]
+    L.lmg_surrogate_gradient_dev.argtypes = [VP, VP, VP, I32, VP, PF64]
+    L.lmg_surrogate_train_dev.argtypes = [VP, VP, VP, I32, I32, I32, F64, U64, VP]
+    L.lmg_surrogate_map_dev.argtypes = [VP, VP, VP, I32, VP, F64, I32, U64, PF64, PI32]
     _lib = L
     return L
 
@@ -127,4 +141,9 @@ EXPORTED = (
     "lmg_sim_device_error", "lmg_sim_step_timed",
     "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
     "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_sync", "lmg_batch_store",
+    "lmg_get_params",
+    "lmg_surrogate_create", "lmg_surrogate_destroy", "lmg_surrogate_last_error", "lmg_surrogate_shape",
+    "lmg_surrogate_get_weights", "lmg_surrogate_set_weights", "lmg_surrogate_occupancy_dev",
+    "lmg_surrogate_infer_dev", "lmg_surrogate_gradient_dev", "lmg_surrogate_train_dev",
+    "lmg_surrogate_map_dev",
 )

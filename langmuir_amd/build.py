@@ -6,7 +6,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "langmuir_gpu.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("langmuir_gpu.hip", "lmg_surrogate.hip")]
 OUT = os.path.join(HERE, "liblangmuir_gpu.so")
 
 HIPCC_FLAGS = [
@@ -21,7 +21,7 @@ def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
     cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-           "-o", tmp, SRC]
+           "-o", tmp, *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

@@ -280,7 +280,7 @@ lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int n
     float2* part = static_cast<float2*>(c->b_part.p);
     if (nc > 0) hipLaunchKernelGGL(to_float4_kernel, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d, d_cs, d_cq, nc, chg);
     hipLaunchKernelGGL(to_float4_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, c->d, d_ts, nullptr, nt, tgt);
-    const uint32_t c2m1 = cutoff_bits(c->p.cutoff, false);
+    const uint32_t c2m1 = cutoff_bits(c->p.cutoff, flags & LMG_NO_CUTOFF);
     const float gk = gauss_k(c->p.gaussian_sigma);
     dim3 grid((nt + kBlock - 1) / kBlock, S);
     const bool gauss = flags & LMG_GAUSS, image = flags & LMG_IMAGE_LEFT;
@@ -437,6 +437,12 @@ void lmg_destroy(lmg_ctx* c) {
 
 void* lmg_stream(lmg_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
 
+lmg_status lmg_get_params(const lmg_ctx* c, lmg_params* out) {
+    if (!c || !out) return LMG_ERR_ARG;
+    *out = c->p;
+    return LMG_OK;
+}
+
 lmg_status lmg_sync(lmg_ctx* c) {
     if (!c) return LMG_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->dev));
@@ -449,7 +455,7 @@ lmg_status lmg_pair_potential_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t
     if (!c) return LMG_ERR_ARG;
     if (nc < 0 || nt < 0 || (nc > 0 && (!d_cs || !d_cq)) || (nt > 0 && (!d_ts || !d_out)))
         return fail(c, LMG_ERR_ARG, "invalid pair_potential arguments");
-    if (flags & ~(LMG_GAUSS | LMG_IMAGE_LEFT)) return fail(c, LMG_ERR_ARG, "unsupported flags");
+    if (flags & ~(LMG_GAUSS | LMG_IMAGE_LEFT | LMG_NO_CUTOFF)) return fail(c, LMG_ERR_ARG, "unsupported flags");
     if ((flags & LMG_GAUSS) && !(c->p.gaussian_sigma > 0))
         return fail(c, LMG_ERR_ARG, "LMG_GAUSS needs gaussian_sigma > 0");
     if (nt == 0) return LMG_OK;

@@ -1,0 +1,145 @@
+"""FP64 numpy restatement of the spectral map surrogate (forward pass, loss and
+gradient) and of the ideal weights it should learn.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ and by the CPU legs of
+``__graft_entry__.smoke()`` / ``bench.py`` as the checker; the product package
+``langmuir_amd`` never imports it.
+
+The operator being learned is the full-lattice potential map of the reference
+(kernel.cl:56-220, coulomb1/gauss1: out[s] = pref * sum_j q_j f(r_sj) over
+0 < r^2 < c^2, f = 1/r or erf(r / (sqrt(2) sigma)) / r), which this repository's
+C++ oracle restates as ``orc_potential_map``.  The surrogate network is
+
+    map = window( irfftn( W * rfftn(q, s=P) , s=P ) )
+
+on a padded grid P (per axis >= N + c - 1, or >= 2N - 1 without a cutoff,
+rounded up to even; 1 for a flat axis), arrays in [z][y][x] order (site
+s = x + X (y + Y z), cubicgrid.cpp:162-165).  ``irfftn`` here is numpy's: the
+imaginary parts of the kx = 0 and kx = Px/2 planes are discarded, exactly the
+half-spectrum convention of the device pipeline.
+
+Parity of these functions is pinned in tests/test_oracle_surrogate.py against
+independent facts: ``ideal_weights`` reproduces ``orc_potential_map`` (C++
+closed form) to rounding; ``loss_and_grad`` agrees with central finite
+differences of ``loss``; the padding rule is what makes the first check exact.
+"""
+from __future__ import annotations
+
+import math
+from typing import Tuple
+
+import numpy as np
+
+
+def padded_size(n: int, cutoff: int, no_cutoff: bool = False) -> int:
+    if n <= 1:
+        return 1
+    m = 2 * n - 1 if no_cutoff else min(2 * n - 1, n + cutoff - 1)
+    return m + (m & 1)
+
+
+def padded_shape(dims, cutoff: int, no_cutoff: bool = False) -> Tuple[int, int, int]:
+    """(Px, Py, Pz) for lattice dims (X, Y, Z)."""
+    return tuple(padded_size(int(n), int(cutoff), no_cutoff) for n in dims)
+
+
+def spectrum_shape(dims, cutoff: int, no_cutoff: bool = False) -> Tuple[int, int, int]:
+    px, py, pz = padded_shape(dims, cutoff, no_cutoff)
+    return (pz, py, px // 2 + 1)
+
+
+def circular_kernel(dims, cutoff: int, pref: float, sigma: float = 0.0, no_cutoff: bool = False) -> np.ndarray:
+    """K[dz % Pz, dy % Py, dx % Px] = pref f(r) for every offset a lattice pair
+    can have, f as in the map (0 < r^2 < c^2 unless no_cutoff)."""
+    X, Y, Z = (int(n) for n in dims)
+    px, py, pz = padded_shape(dims, cutoff, no_cutoff)
+    dz = np.arange(-(Z - 1), Z)[:, None, None]
+    dy = np.arange(-(Y - 1), Y)[None, :, None]
+    dx = np.arange(-(X - 1), X)[None, None, :]
+    r2 = (dx * dx + dy * dy + dz * dz).astype(np.int64)
+    inside = r2 > 0
+    if not no_cutoff:
+        inside &= r2 < int(cutoff) * int(cutoff)
+    r = np.sqrt(np.where(inside, r2, 1).astype(np.float64))
+    if sigma > 0:
+        f = np.array([math.erf(v) for v in (r / (math.sqrt(2.0) * sigma)).ravel()]).reshape(r.shape) / r
+    else:
+        f = 1.0 / r
+    vals = np.where(inside, pref * f, 0.0)
+    K = np.zeros((pz, py, px))
+    np.add.at(K, (np.broadcast_to(dz % pz, vals.shape), np.broadcast_to(dy % py, vals.shape),
+                  np.broadcast_to(dx % px, vals.shape)), vals)
+    return K
+
+
+def ideal_weights(dims, cutoff: int, pref: float, sigma: float = 0.0, no_cutoff: bool = False) -> np.ndarray:
+    """The weights that make the network equal the exact map: rfftn of the
+    circular kernel."""
+    return np.fft.rfftn(circular_kernel(dims, cutoff, pref, sigma, no_cutoff), axes=(0, 1, 2))
+
+
+def _grid(q: np.ndarray, dims) -> np.ndarray:
+    X, Y, Z = (int(n) for n in dims)
+    return np.asarray(q, np.float64).reshape(-1, Z, Y, X)
+
+
+def forward(W: np.ndarray, occ: np.ndarray, dims, cutoff: int, no_cutoff: bool = False) -> np.ndarray:
+    """Maps (batch, V) of occupancy/charge grids (batch, V)."""
+    X, Y, Z = (int(n) for n in dims)
+    px, py, pz = padded_shape(dims, cutoff, no_cutoff)
+    q = _grid(occ, dims)
+    spec = np.fft.rfftn(q, s=(pz, py, px), axes=(1, 2, 3))
+    out = np.fft.irfftn(W[None] * spec, s=(pz, py, px), axes=(1, 2, 3))
+    return out[:, :Z, :Y, :X].reshape(q.shape[0], -1)
+
+
+def loss(W, occ, target, dims, cutoff, no_cutoff=False) -> float:
+    t = np.asarray(target, np.float64).reshape(-1, int(np.prod(dims)))
+    r = forward(W, occ, dims, cutoff, no_cutoff) - t
+    return float(np.sum(r * r) / np.sum(t * t))
+
+
+def loss_and_grad(W, occ, target, dims, cutoff, no_cutoff=False):
+    """(L, dL/dW) with dL/dW = dL/dRe W + i dL/dIm W.
+
+    With V = window(irfftn(Y)), Y = W X: dV[n]/dRe Y_k = c_k cos(theta)/P and
+    dV[n]/dIm Y_k = -c_k sin(theta)/P (theta = 2 pi k.n / P, c_k = 1 on the
+    kx = 0 and Nyquist planes, 2 elsewhere), so G_k = dL/dRe Y_k + i dL/dIm Y_k
+    = (c_k / P) rfftn(pad(g))_k with g = dL/dV, and dL/dW = G conj(X)."""
+    X_, Y_, Z_ = (int(n) for n in dims)
+    px, py, pz = padded_shape(dims, cutoff, no_cutoff)
+    q = _grid(occ, dims)
+    t = np.asarray(target, np.float64).reshape(q.shape)
+    spec = np.fft.rfftn(q, s=(pz, py, px), axes=(1, 2, 3))
+    full = np.fft.irfftn(W[None] * spec, s=(pz, py, px), axes=(1, 2, 3))
+    r = full[:, :Z_, :Y_, :X_] - t
+    den = float(np.sum(t * t))
+    L = float(np.sum(r * r)) / den
+    g = np.zeros((q.shape[0], pz, py, px))
+    g[:, :Z_, :Y_, :X_] = 2.0 * r / den
+    ck = np.full(px // 2 + 1, 2.0)
+    ck[0] = 1.0
+    if px % 2 == 0:
+        ck[-1] = 1.0
+    if px == 1:
+        ck[:] = 1.0
+    G = ck[None, None, None, :] / (px * py * pz) * np.fft.rfftn(g, axes=(1, 2, 3))
+    return L, np.sum(G * np.conj(spec), axis=0)
+
+
+def truncate_modes(W: np.ndarray, modes: int) -> np.ndarray:
+    """Keep |k| < modes on every axis (an FNO layer's retained corners)."""
+    pz, py, kx = W.shape
+    keep = np.zeros(W.shape, bool)
+    kz = np.minimum(np.arange(pz), pz - np.arange(pz)) < modes
+    ky = np.minimum(np.arange(py), py - np.arange(py)) < modes
+    kxx = np.arange(kx) < modes
+    keep[np.ix_(kz, ky, kxx)] = True
+    return np.where(keep, W, 0)
+
+
+def occupancy(dims, sites, charges) -> np.ndarray:
+    """int8 charge sum per site."""
+    occ = np.zeros(int(np.prod(dims)), np.int64)
+    np.add.at(occ, np.asarray(sites, np.int64), np.asarray(charges, np.int64))
+    return occ.astype(np.int8)
