@@ -15,6 +15,9 @@ Prints ONE JSON line.  ``value`` is KMC steps/s; ``pair_evals_per_s`` counts
 ``roofline`` is for the dominant kernel (de_kernel), FP32 VALU bound:
 11 flops per in-cutoff pair term, terms counted exactly on the timed state.
 ``map`` times the full-lattice potential map (256x256x64, 50,000 charges).
+``batch`` advances 256 independent replicas of uneven size together.
+``surrogate`` trains the learned map surrogate on snapshots of the solar run
+and times its gated map against the exact map kernel on held-out charges.
 ``cpu_baseline`` times the FP64 oracle (a restatement of the reference CPU path,
 OpenMP over carriers like QtConcurrent::map) on a bounded sample of the same
 workload.
@@ -131,23 +134,6 @@ class HipTimer:
         return float(ms.value)
 
 
-class DeviceArray:
-    """Device buffer from the HIP runtime the library itself uses (a second HIP
-    runtime in the process, e.g. torch's bundled one, would not see the GPU)."""
-
-    def __init__(self, host: np.ndarray):
-        self.hip = C.CDLL("libamdhip64.so")
-        self.ptr = C.c_void_p()
-        self.nbytes = host.nbytes
-        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.nbytes, 4))) == 0
-        assert self.hip.hipMemcpy(self.ptr, host.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), 1) == 0
-
-    def free(self):
-        if self.ptr:
-            self.hip.hipFree(self.ptr)
-            self.ptr = C.c_void_p()
-
-
 def bench_map(reps=3):
     """Full-lattice map, 256x256x64, 50,000 charges (+/-1 equal mix), cutoff 50 and none."""
     import langmuir_amd as L
@@ -159,8 +145,8 @@ def bench_map(reps=3):
     cq = np.where(np.arange(n) < n // 2, 1, -1).astype(np.int32)
     out = {}
     ctx = L.Context(dims, cutoff=50)
-    d_cs, d_cq = DeviceArray(cs), DeviceArray(cq)
-    d_out = DeviceArray(np.zeros(V, np.float32))
+    d_cs, d_cq = L.DeviceBuffer.from_array(cs), L.DeviceBuffer.from_array(cq)
+    d_out = L.DeviceBuffer(4 * V)
     timer = HipTimer(ctx.stream)
     for name, flags in (("cutoff50", 0), ("nocutoff", L.LMG_NO_CUTOFF)):
         ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags)
@@ -231,6 +217,93 @@ def bench_batch(steps=200, warmup=20, n_rep=256):
     return out
 
 
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_gemm_shapes(dims, padded):
+    """(M, N, K, batch) of the six DFT GEMMs of one surrogate map."""
+    X, Y, Z = dims
+    px, py, pz = padded
+    kx2 = 2 * (px // 2 + 1)
+    return [(Z * Y, kx2, X, 1), (py, kx2, 2 * Y, Z), (pz, py * kx2, 2 * Z, 1),
+            (Z, py * kx2, 2 * pz, 1), (Y, kx2, 2 * py, Z), (Z * Y, X, kx2, 1)]
+
+
+def bench_surrogate(workload, n_snap=64, stride=25, iters=1200, reps=50):
+    """Learned map surrogate on the solar lattice (128x128x32, cutoff 30):
+    snapshots of the KMC run on the device labelled by the exact map kernel,
+    training on the device, then held-out accuracy and gated-map throughput
+    against the exact map on the same charges."""
+    import langmuir_amd as L
+    p, e, h, traps, pot = workload
+    dims = (p.grid_x, p.grid_y, p.grid_z)
+    ctx = L.Context.from_parameters(p)
+    sur = L.Surrogate(ctx, max_batch=8)
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, context=ctx)
+    sim.step(200)
+    snaps = L.SnapshotSet(sur, n_snap)
+    t0 = time.perf_counter()
+    snaps.add_simulation(sim, n_snap, stride)
+    t_data = time.perf_counter() - t0
+    n_train = n_snap - 8
+    t0 = time.perf_counter()
+    hist = snaps.train(batch=8, iterations=iters, momentum=0.9, seed=1, first=0, count=n_train)
+    t_train = time.perf_counter() - t0
+    errs = []
+    for i in range(n_train, n_snap):
+        got = sur.infer(snaps.occupancy(i))[0]
+        t = snaps.targets(i).astype(np.float64)
+        errs.append(float(np.linalg.norm(got - t) / np.linalg.norm(t)))
+    occ = snaps.occupancy(n_snap - 1).astype(np.int64)
+    sites = np.nonzero(occ)[0].astype(np.int32)
+    q = occ[sites].astype(np.int32)
+    d_s, d_q = L.DeviceBuffer.from_array(sites), L.DeviceBuffer.from_array(q)
+    d_out = L.DeviceBuffer(4 * sur.V)
+    d_occ = L.DeviceBuffer(sur.V)
+    sur.occupancy_dev(d_s.ptr, d_q.ptr, len(sites), d_occ.ptr)
+    timer = HipTimer(ctx.stream)
+
+    def timed(fn, n):
+        fn()
+        ctx.sync()
+        timer.start()
+        for _ in range(n):
+            fn()
+        return timer.stop_ms() / n
+
+    ms_exact = timed(lambda: ctx.potential_map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 0), 5)
+    ms_gated = timed(lambda: sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3, wait=False), reps)
+    gate_err, used = sur.last_gate()
+    full_err = float(np.linalg.norm(d_out.download(np.float32, sur.V) - snaps.targets(n_snap - 1))
+                     / np.linalg.norm(snaps.targets(n_snap - 1)))
+    ms_infer = timed(lambda: sur.infer_dev(d_occ.ptr, 1, d_out.ptr), reps)
+    shapes = surrogate_gemm_shapes(dims, sur.padded)
+    alg = sum(2.0 * m * n * k * b for m, n, k, b in shapes)
+    executed = alg * 3 - 2.0 * shapes[0][0] * shapes[0][1] * shapes[0][2]   # int8 stage: two products
+    tflops = executed / (ms_infer * 1e-3) / 1e12
+    pz, py, kxn = sur.spectrum_shape
+    free = (pz // 2 + 1) * (py // 2 + 1) * kxn
+    log(f"surrogate: {n_snap} snapshots in {t_data:.1f} s, {iters} steps in {t_train:.1f} s (loss "
+        f"{hist[0]:.2e} -> {hist[-1]:.2e}); held-out rel RMS max {max(errs):.2e}; gated map {ms_gated * 1e3:.1f} us "
+        f"(gate {gate_err:.2e}, exact used {used}), infer {ms_infer * 1e3:.1f} us, exact map {ms_exact * 1e3:.1f} us "
+        f"-> {ms_exact / ms_gated:.1f}x")
+    return {
+        "maps_per_s": round(1e3 / ms_gated, 2), "exact_maps_per_s": round(1e3 / ms_exact, 2),
+        "speedup_vs_exact": round(ms_exact / ms_gated, 2),
+        "gated_map_us": round(ms_gated * 1e3, 2), "infer_us": round(ms_infer * 1e3, 2),
+        "exact_map_us": round(ms_exact * 1e3, 2),
+        "rel_rms_heldout_max": max(errs), "gate_error": gate_err, "gate_sites": 2048, "used_exact": used,
+        "rel_rms_gated_map": full_err,
+        "weights_stored": 2 * sur.n_weights, "weights_free": free, "padded": list(sur.padded),
+        "snapshots": n_snap, "snapshot_stride_steps": stride, "train_snapshots": n_train,
+        "train_iterations": iters, "train_s": round(t_train, 3), "loss_first_last": [float(hist[0]), float(hist[-1])],
+        "mfma": {"bound": "mfma", "algorithmic_gflop_per_map": round(alg / 1e9, 4),
+                 "executed_bf16_gflop_per_map": round(executed / 1e9, 4),
+                 "achieved_tflops": round(tflops, 2), "peak_tflops": BF16_PEAK_TFLOPS,
+                 "frac": round(tflops / BF16_PEAK_TFLOPS, 5)},
+    }
+
+
 def cpu_baseline(workload, budget_s=15.0):
     """FP64 oracle (restated reference CPU path) on the same workload, all host cores."""
     from oracle import pyoracle as O
@@ -275,6 +348,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -350,7 +424,8 @@ def main():
         res["map"] = bench_map()
     if not args.no_batch:
         res["batch"] = bench_batch()
-    res["surrogate"] = None
+    if not args.no_surrogate and args.workload == "solar":
+        res["surrogate"] = bench_surrogate(workload)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(workload, args.cpu_budget)
         res["cpu_baseline"] = cb

@@ -104,6 +104,13 @@ lmg_status lmg_potential_map(lmg_ctx* ctx, const int32_t* charge_site,
 lmg_status lmg_potential_map_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
                                  const int32_t* d_charge_q, int32_t n_charges,
                                  int32_t flags, float* d_out);
+/* lmg_potential_map_dev that runs only if *d_run != 0 when the device reaches
+ * it (a condition computed earlier on the stream); d_out is untouched
+ * otherwise.  No counterpart in the reference: lets the surrogate's error gate
+ * fall back to the exact map without a host round trip. */
+lmg_status lmg_potential_map_if_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
+                                    const int32_t* d_charge_q, int32_t n_charges,
+                                    int32_t flags, const int32_t* d_run, float* d_out);
 
 /* ---------------------------------------------------------------------------
  * KMC sweep on the device (Simulation::performIterations with the Coulomb
@@ -248,12 +255,21 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
                                    int32_t batch, int32_t iterations, double momentum, uint64_t seed,
                                    float* loss_history);
 /* Gated map of a charge list: surrogate map, exact pair sums at gate_sites
- * sampled sites (seeded), *gate_error = relative RMS error over them; when it
- * exceeds tolerance the exact map (lmg_potential_map_dev) overwrites d_map and
- * *used_exact = 1.  Synchronous (the gate decision is taken on the host). */
+ * sampled sites (seeded), gate error = relative RMS error over them; when it
+ * exceeds tolerance the exact map (lmg_potential_map_if_dev) overwrites d_map.
+ * The decision is taken on the device.  With gate_error and used_exact both
+ * NULL the call is asynchronous on the context stream (lmg_surrogate_last_gate
+ * reads the outcome later); otherwise it waits and reports the error and
+ * whether the exact map was used. */
 lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n, float* d_map,
                                  double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
                                  int32_t* used_exact);
+/* Outcome of the latest lmg_surrogate_map_dev (waits for the stream). */
+lmg_status lmg_surrogate_last_gate(lmg_surrogate* s, double* gate_error, int32_t* used_exact);
+/* lmg_surrogate_map_dev records its launches once as a HIP graph and replays
+ * it while the pointers, sizes and tolerance stay the same (default on); off
+ * launches every kernel each call. */
+lmg_status lmg_surrogate_set_graphs(lmg_surrogate* s, int32_t on);
 
 #ifdef __cplusplus
 }

@@ -22,151 +22,12 @@
 
 #include "langmuir_gpu.h"
 #include "lmg_device.h"
+#include "lmg_pairsum.h"
 #include "lmg_sim.h"
 
 using namespace lmg;
 
 namespace {
-
-constexpr int kBlock = 256;      // pair / dE / map workgroup
-constexpr int kTile = 256;       // charges per LDS tile
-constexpr int kBlk = 32;         // terms summed in plain FP32 before a compensated add
-constexpr int kMapT = 4;         // lattice sites per map thread (along x)
-
-__device__ __forceinline__ float rsq(float r2) { return __builtin_amdgcn_rsqf(r2); }
-__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// 0 < r2 < c2 for non-negative floats, as one unsigned compare on the bits
-__device__ __forceinline__ bool in_cut(float r2, uint32_t c2bits_m1) {
-    return (__float_as_uint(r2) - 1u) < c2bits_m1;
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-    float y = x - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
-// ===========================================================================
-// (site, q) -> float4 (x, y, z, q)
-// ===========================================================================
-__global__ void to_float4_kernel(Dims d, const int* __restrict__ site, const int* __restrict__ q,
-                                 int n, float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int x, y, z;
-    decode(d, site[i], x, y, z);
-    out[i] = make_float4(float(x), float(y), float(z), q ? float(q[i]) : 0.f);
-}
-
-// ===========================================================================
-// Pair sums at explicit targets.  grid = (ceil(nt / 256), S); chunk s covers
-// charges [s*chunk, (s+1)*chunk).  part[s*ld + t] = (sum, compensation).
-// ===========================================================================
-template <bool kGauss, bool kImage>
-__global__ __launch_bounds__(kBlock) void pair_kernel(const float4* __restrict__ tgt, int nt,
-                                                      const float4* __restrict__ chg, int nc,
-                                                      uint32_t c2m1, float gk,
-                                                      float2* __restrict__ part, int ld) {
-    __shared__ float4 tile[kTile];
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int S = gridDim.y;
-    const int chunk = (nc + S - 1) / S;
-    const int j0 = blockIdx.y * chunk, j1 = min(nc, j0 + chunk);
-    const float4 me = t < nt ? tgt[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float s = 0.f, c = 0.f;
-    for (int base = j0; base < j1; base += kTile) {
-        const int n = min(kTile, j1 - base);
-        __syncthreads();
-        if (threadIdx.x < n) tile[threadIdx.x] = chg[base + threadIdx.x];
-        __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += kBlk) {
-            const int k1 = min(n, k0 + kBlk);
-            float acc = 0.f;
-            for (int k = k0; k < k1; ++k) {
-                const float4 q = tile[k];
-                const float dx = kImage ? (me.x + q.x + 1.f) : (me.x - q.x);
-                const float dy = me.y - q.y, dz = me.z - q.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float a = rsq(r2);
-                float f = a;
-                if (kGauss) f = erff(gk * (r2 * a)) * a;
-                acc = fmaf(q.w, in_cut(r2, c2m1) ? f : 0.f, acc);
-            }
-            kahan_add(s, c, acc);
-        }
-    }
-    if (t < nt) part[blockIdx.y * ld + t] = make_float2(s, -c);
-}
-
-__global__ void pair_reduce_kernel(const float2* __restrict__ part, int nt, int S, int ld,
-                                   double scale, double* __restrict__ out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    double v = 0.0;
-    for (int s = 0; s < S; ++s) {
-        float2 p = part[s * ld + t];
-        v += double(p.x) + double(p.y);
-    }
-    out[t] = scale * v;
-}
-
-// ===========================================================================
-// Full-lattice map.  A thread owns kMapT consecutive x sites of one (y, z)
-// row; a workgroup owns (tx * kMapT) x (256 / tx) sites of one z plane.
-// ===========================================================================
-template <bool kGauss>
-__global__ __launch_bounds__(kBlock) void map_kernel(Dims d, const float4* __restrict__ chg, int nc,
-                                                     uint32_t c2m1, float gk, float pref, int tx_n,
-                                                     float* __restrict__ out) {
-    __shared__ float4 tile[kTile];
-    const int tx = threadIdx.x % tx_n, ty = threadIdx.x / tx_n;
-    const int ty_n = kBlock / tx_n;
-    const int x0 = (blockIdx.x * tx_n + tx) * kMapT;
-    const int y = blockIdx.y * ty_n + ty;
-    const int z = blockIdx.z;
-    const float fy = float(y), fz = float(z);
-    float s[kMapT], c[kMapT];
-#pragma unroll
-    for (int u = 0; u < kMapT; ++u) { s[u] = 0.f; c[u] = 0.f; }
-    for (int base = 0; base < nc; base += kTile) {
-        const int n = min(kTile, nc - base);
-        __syncthreads();
-        if (threadIdx.x < n) tile[threadIdx.x] = chg[base + threadIdx.x];
-        __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += kBlk) {
-            const int k1 = min(n, k0 + kBlk);
-            float acc[kMapT];
-#pragma unroll
-            for (int u = 0; u < kMapT; ++u) acc[u] = 0.f;
-            for (int k = k0; k < k1; ++k) {
-                const float4 q = tile[k];
-                const float dy = fy - q.y, dz = fz - q.z;
-                const float dyz = dy * dy + dz * dz;
-                const float dx0 = float(x0) - q.x;
-#pragma unroll
-                for (int u = 0; u < kMapT; ++u) {
-                    const float dx = dx0 + float(u);
-                    const float r2 = fmaf(dx, dx, dyz);
-                    const float a = rsq(r2);
-                    float f = a;
-                    if (kGauss) f = erff(gk * (r2 * a)) * a;
-                    acc[u] = fmaf(q.w, in_cut(r2, c2m1) ? f : 0.f, acc[u]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kMapT; ++u) kahan_add(s[u], c[u], acc[u]);
-        }
-    }
-    if (y < d.Y) {
-#pragma unroll
-        for (int u = 0; u < kMapT; ++u) {
-            const int x = x0 + u;
-            if (x < d.X) out[encode(d, x, y, z)] = pref * (s[u] - c[u]);
-        }
-    }
-}
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -238,23 +99,6 @@ lmg_status use_device(lmg_ctx* c) {
 }
 
 // c^2 - 1 as float bits for the 0 < r^2 < c^2 test
-uint32_t cutoff_bits(int cutoff, bool none) {
-    float c2 = none ? INFINITY : float(double(cutoff) * double(cutoff));
-    uint32_t b;
-    std::memcpy(&b, &c2, 4);
-    return b - 1u;
-}
-
-float gauss_k(double sigma) { return sigma > 0 ? float(1.0 / (std::sqrt(2.0) * sigma)) : 0.f; }
-
-int split_count(int nt, int nc) {
-    // enough waves to cover 256 CUs several times over, >= 64 charges per chunk
-    const int waves = std::max(1, (nt + 63) / 64);
-    int S = (4096 + waves - 1) / waves;
-    S = std::min(S, std::max(1, nc / 64));
-    return std::max(1, std::min(S, 128));
-}
-
 lmg_status launch_check(lmg_ctx* c, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, LMG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -293,15 +137,19 @@ lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int n
     else
         hipLaunchKernelGGL((pair_kernel<false, false>), grid, dim3(kBlock), 0, c->stream, tgt, nt, chg, nc, c2m1, gk, part, nt);
     const double scale = (image ? -1.0 : 1.0) * c->p.prefactor;
-    hipLaunchKernelGGL(pair_reduce_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, part, nt, S, nt, scale, d_out);
+    hipLaunchKernelGGL(pair_reduce_kernel, dim3((nt + kBlock / kRedLanes - 1) / (kBlock / kRedLanes)), dim3(kBlock), 0,
+                       c->stream, part, nt, S, nt, scale, d_out);
     return launch_check(c, "pair_kernel");
 }
 
-lmg_status map_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, int flags, float* d_out) {
+lmg_status map_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, int flags, float* d_out,
+                    const int32_t* d_run = nullptr) {
     lmg_status st;
     if ((st = ensure(c, c->b_chg, size_t(std::max(nc, 1)) * sizeof(float4))) != LMG_OK) return st;
     float4* chg = static_cast<float4*>(c->b_chg.p);
-    if (nc > 0) hipLaunchKernelGGL(to_float4_kernel, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d, d_cs, d_cq, nc, chg);
+    if (nc > 0)
+        hipLaunchKernelGGL(to_float4_kernel, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d, d_cs, d_cq, nc,
+                           chg, d_run);
     int tx = 1;
     while (tx < 64 && tx * kMapT < c->d.X) tx <<= 1;
     const int ty = kBlock / tx;
@@ -310,9 +158,11 @@ lmg_status map_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc
     const float gk = gauss_k(c->p.gaussian_sigma);
     const float pref = float(c->p.prefactor);
     if (flags & LMG_GAUSS)
-        hipLaunchKernelGGL(map_kernel<true>, grid, dim3(kBlock), 0, c->stream, c->d, chg, nc, c2m1, gk, pref, tx, d_out);
+        hipLaunchKernelGGL(map_kernel<true>, grid, dim3(kBlock), 0, c->stream, c->d, chg, nc, c2m1, gk, pref, tx,
+                           d_run, d_out);
     else
-        hipLaunchKernelGGL(map_kernel<false>, grid, dim3(kBlock), 0, c->stream, c->d, chg, nc, c2m1, gk, pref, tx, d_out);
+        hipLaunchKernelGGL(map_kernel<false>, grid, dim3(kBlock), 0, c->stream, c->d, chg, nc, c2m1, gk, pref, tx,
+                           d_run, d_out);
     return launch_check(c, "map_kernel");
 }
 
@@ -503,6 +353,19 @@ lmg_status lmg_potential_map_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t*
     lmg_status st;
     if ((st = use_device(c)) != LMG_OK) return st;
     return map_core(c, d_cs, d_cq, nc, flags, d_out);
+}
+
+lmg_status lmg_potential_map_if_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int32_t nc,
+                                    int32_t flags, const int32_t* d_run, float* d_out) {
+    if (!c) return LMG_ERR_ARG;
+    if (nc < 0 || (nc > 0 && (!d_cs || !d_cq)) || !d_out || !d_run)
+        return fail(c, LMG_ERR_ARG, "invalid map arguments");
+    if (flags & ~(LMG_GAUSS | LMG_NO_CUTOFF)) return fail(c, LMG_ERR_ARG, "unsupported flags");
+    if ((flags & LMG_GAUSS) && !(c->p.gaussian_sigma > 0))
+        return fail(c, LMG_ERR_ARG, "LMG_GAUSS needs gaussian_sigma > 0");
+    lmg_status st;
+    if ((st = use_device(c)) != LMG_OK) return st;
+    return map_core(c, d_cs, d_cq, nc, flags, d_out, d_run);
 }
 
 lmg_status lmg_potential_map(lmg_ctx* c, const int32_t* cs, const int32_t* cq, int32_t nc, int32_t flags,

@@ -14,7 +14,7 @@
 //
 // The constant operand is stored pre-split in global memory (hi and lo BF16
 // planes), k contiguous: Ac as [m][k] for mode 1, Bc transposed as [n][k] for
-// mode 0, rows padded to a multiple of 64 and k to a multiple of kTK with zeros.
+// mode 0, rows padded to a multiple of 64 and k to a multiple of 32 with zeros.
 // Each wave reads its constant fragments straight from global memory (they stay
 // in L2); the data tile goes through LDS, k contiguous, double-buffered.
 #pragma once
@@ -27,7 +27,8 @@ namespace lmg {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-constexpr int kTM = 64, kTN = 64, kTK = 32;   // block tile
+constexpr int kTM = 64, kTN = 64, kTK = 32;   // block tile; a K tile is kSlabs slabs of 32
+constexpr int kSlabs = kTK / 32;
 constexpr int kLds = kTK + 8;                 // LDS row stride (bf16): 80 bytes, 16-byte aligned rows
 
 // round-to-nearest-even FP32 -> BF16 bits
@@ -61,14 +62,14 @@ struct GemmArgs {
     float alpha;
 };
 
-// One data tile's share of a thread: 4 float pairs.
+// One data tile's share of a thread: 4 float pairs per slab of 32 k.
 struct DataRegs {
-    float2 v[4];
+    float2 v[kSlabs][4];
 };
 
 template <int kMode, typename TD, bool kRot>
-__device__ __forceinline__ void load_data(const GemmArgs& g, const TD* __restrict__ D, int m0, int n0, int k0,
-                                          DataRegs& r) {
+__device__ __forceinline__ void load_slab(const GemmArgs& g, const TD* __restrict__ D, int m0, int n0, int k0,
+                                          float2 (&r)[4]) {
     const int t = threadIdx.x;
     if (kMode == 0) {
         // 64 rows (m) x 32 k: pair p = t + 256 e -> row p / 16, k pair p % 16
@@ -81,7 +82,7 @@ __device__ __forceinline__ void load_data(const GemmArgs& g, const TD* __restric
                 if (k + 1 < g.K) v = make_float2(float(src[0]), float(src[1]));
                 else if (k < g.K) v.x = float(src[0]);
             }
-            r.v[e] = v;
+            r[e] = v;
         }
     } else {
         // 32 k x 64 n: thread -> column pair t % 32, k row t / 32 + 8 e
@@ -97,50 +98,72 @@ __device__ __forceinline__ void load_data(const GemmArgs& g, const TD* __restric
                     v = make_float2(-u.y, u.x);
                 }
             }
-            r.v[e] = v;
+            r[e] = v;
         }
     }
+}
+
+template <int kMode, typename TD, bool kRot>
+__device__ __forceinline__ void load_data(const GemmArgs& g, const TD* __restrict__ D, int m0, int n0, int k0,
+                                          DataRegs& r) {
+#pragma unroll
+    for (int sl = 0; sl < kSlabs; ++sl)
+        if (k0 + 32 * sl < g.K) load_slab<kMode, TD, kRot>(g, D, m0, n0, k0 + 32 * sl, r.v[sl]);
+        else
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r.v[sl][e] = make_float2(0.f, 0.f);
 }
 
 // registers -> LDS (hi, lo planes), k contiguous per row; mode 1 transposes
 template <int kMode>
 __device__ __forceinline__ void store_data(const DataRegs& r, uint16_t (*hi)[kLds], uint16_t (*lo)[kLds]) {
     const int t = threadIdx.x;
-    if (kMode == 0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int p = t + 256 * e, row = p >> 4, kk = 2 * (p & 15);
-            uint32_t h, l;
-            split2(r.v[e].x, r.v[e].y, h, l);
-            *reinterpret_cast<uint32_t*>(&hi[row][kk]) = h;
-            *reinterpret_cast<uint32_t*>(&lo[row][kk]) = l;
-        }
-    } else {
-        const int np = t & 31;
+    for (int sl = 0; sl < kSlabs; ++sl) {
+        if (kMode == 0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int kr = (t >> 5) + 8 * e;
-            uint32_t h, l;
-            split2(r.v[e].x, r.v[e].y, h, l);   // (col 2 np, col 2 np + 1) at row kr
-            hi[2 * np][kr] = uint16_t(h & 0xffffu);
-            hi[2 * np + 1][kr] = uint16_t(h >> 16);
-            lo[2 * np][kr] = uint16_t(l & 0xffffu);
-            lo[2 * np + 1][kr] = uint16_t(l >> 16);
+            for (int e = 0; e < 4; ++e) {
+                const int p = t + 256 * e, row = p >> 4, kk = 32 * sl + 2 * (p & 15);
+                uint32_t h, l;
+                split2(r.v[sl][e].x, r.v[sl][e].y, h, l);
+                *reinterpret_cast<uint32_t*>(&hi[row][kk]) = h;
+                *reinterpret_cast<uint32_t*>(&lo[row][kk]) = l;
+            }
+        } else {
+            const int np = t & 31;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int kr = 32 * sl + (t >> 5) + 8 * e;
+                uint32_t h, l;
+                split2(r.v[sl][e].x, r.v[sl][e].y, h, l);   // (col 2 np, col 2 np + 1) at row kr
+                hi[2 * np][kr] = uint16_t(h & 0xffffu);
+                hi[2 * np + 1][kr] = uint16_t(h >> 16);
+                lo[2 * np][kr] = uint16_t(l & 0xffffu);
+                lo[2 * np + 1][kr] = uint16_t(l >> 16);
+            }
         }
     }
 }
 
+constexpr int kSteps = kTK / 16;   // MFMA k-steps per K tile
+
 struct ConstFrag {
-    bf16x8_t hi[2], lo[2];   // the tile's two k-steps of 16
+    bf16x8_t hi[kSteps], lo[kSteps];
 };
 
+// the tile's constant fragments; steps past the padded row length are zero
 __device__ __forceinline__ void load_const(const ConstOperand& c, int row, int k0, ConstFrag& f) {
     const int lane = threadIdx.x & 63;
     const long long base = (long long)row * c.ld + k0 + 8 * (lane >> 5);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        f.hi[s] = *reinterpret_cast<const bf16x8_t*>(c.hi + base + 16 * s);
-        f.lo[s] = *reinterpret_cast<const bf16x8_t*>(c.lo + base + 16 * s);
+    for (int s = 0; s < kSteps; ++s) {
+        if (k0 + 16 * s < c.ld) {
+            f.hi[s] = *reinterpret_cast<const bf16x8_t*>(c.hi + base + 16 * s);
+            f.lo[s] = *reinterpret_cast<const bf16x8_t*>(c.lo + base + 16 * s);
+        } else {
+            f.hi[s] = bf16x8_t{};
+            f.lo[s] = bf16x8_t{};
+        }
     }
 }
 
@@ -183,7 +206,8 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g) {
         const ConstFrag cur = cf;
         if (more) load_const(g.c, crow, (t + 1) * kTK, cf);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < kSteps; ++s) {
+            if (t * kTK + 16 * s >= g.K) break;
             const bf16x8_t dh = *reinterpret_cast<const bf16x8_t*>(&s_hi[buf][drow][16 * s + kq]);
             const bf16x8_t dl = *reinterpret_cast<const bf16x8_t*>(&s_lo[buf][drow][16 * s + kq]);
             if (kMode == 0) {

@@ -32,7 +32,9 @@
 #include <vector>
 
 #include "langmuir_gpu.h"
+#include "lmg_device.h"
 #include "lmg_gemm.h"
+#include "lmg_pairsum.h"
 
 namespace {
 
@@ -44,6 +46,7 @@ constexpr int kPowerIters = 12;   // power iterations for the curvature estimate
 constexpr double kStepFactor = 0.9;
 constexpr int kPrecondSnapshots = 256;   // occupancies averaged into the preconditioner
 constexpr double kPrecondFloor = 1e-3;   // relative floor of the spectral power
+constexpr int kGateMaxSplit = 128;       // charge chunks of the gate's pair sums (split_count's cap)
 
 // Mirror images of spectrum index k = (kz Py + ky) Kx + kx under ky -> -ky and
 // kz -> -kz (the lattice reflections y -> -y, z -> -z; x -> -x is the complex
@@ -254,12 +257,13 @@ __global__ void gate_site_kernel(int32_t* site, int n, long long V, unsigned lon
     if (i < n) site[i] = int32_t(splitmix64(seed * 0x100000001b3ull + (unsigned long long)i) % (unsigned long long)V);
 }
 
-// out[0] = sum (map[site] - exact)^2, out[1] = sum exact^2 (one block)
-__global__ void gate_err_kernel(const float* __restrict__ map, const int32_t* __restrict__ site,
-                                const double* __restrict__ exact, int n, double* __restrict__ out) {
+// part[2 blk] = sum (map[site] - exact)^2, part[2 blk + 1] = sum exact^2 over
+// the block's grid-stride share of the gate sites
+__global__ void gate_part_kernel(const float* __restrict__ map, const int32_t* __restrict__ site,
+                                 const double* __restrict__ exact, int n, double* __restrict__ part) {
     __shared__ double sh[kRedThreads];
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double e = exact[i];
         const double d = double(map[site[i]]) - e;
         a += d * d;
@@ -268,8 +272,27 @@ __global__ void gate_err_kernel(const float* __restrict__ map, const int32_t* __
     a = block_sum(a, sh);
     b = block_sum(b, sh);
     if (threadIdx.x == 0) {
-        out[0] = a;
-        out[1] = b;
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// err = sqrt(sum d^2 / sum e^2) over the gate sites; *run = err > tol (the
+// exact map must replace the surrogate's)
+__global__ void gate_final_kernel(const double* __restrict__ part, int nblk, double tol, double* __restrict__ err,
+                                  int32_t* __restrict__ run) {
+    __shared__ double sh[kRedThreads];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        const double e = b > 0 ? sqrt(a / b) : (a > 0 ? INFINITY : 0.0);
+        *err = e;
+        *run = (e <= tol) ? 0 : 1;
     }
 }
 
@@ -298,9 +321,34 @@ struct lmg_surrogate {
     int* bad = nullptr;
     float* loss = nullptr;
     int loss_cap = 0;
-    int32_t* gate_site = nullptr;
-    double* gate_exact = nullptr;
+    // error gate and exact fallback (own buffers, so a captured graph never
+    // points into memory another call may reallocate)
+    lmg::Dims dims{};
+    int32_t* gate_site = nullptr;  // sampled sites
+    float4* gate_tgt = nullptr;    // their coordinates
+    double* gate_exact = nullptr;  // exact potential there
+    float2* gate_part = nullptr;   // pair-sum partials, kGateMaxSplit x gate_cap
     int gate_cap = 0;
+    int gate_n = 0;                // sites currently in gate_site, and their seed
+    uint64_t gate_seed = 0;
+    float4* chg = nullptr;         // charges as (x, y, z, q)
+    int chg_cap = 0;
+    double* gate_err = nullptr;    // device: latest gate error
+    int32_t* gate_run = nullptr;   // device: latest decision (1 = exact map used)
+    // the gated map as a replayable graph, valid while its key holds
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    unsigned buf_gen = 0;          // bumped whenever a gate/charge buffer is reallocated
+    struct Key {
+        const void *site, *q, *map;
+        int n, gate_sites;
+        double tol;
+        unsigned gen;
+        bool operator==(const Key& o) const {
+            return site == o.site && q == o.q && map == o.map && n == o.n && gate_sites == o.gate_sites &&
+                   tol == o.tol && gen == o.gen;
+        }
+    } gkey{};
     std::vector<void*> allocs;
 };
 
@@ -498,9 +546,9 @@ double host_bf16_value(uint16_t b) {
     return f;
 }
 
-// pad rows to a multiple of 64 and k to a multiple of kTK, split into hi/lo
+// pad rows to a multiple of 64 and k to a multiple of 32, split into hi/lo
 lmg_status upload_const(lmg_surrogate* s, const HostMat& m, lmg::ConstOperand* c) {
-    const int rows = (m.rows + 63) / 64 * 64, ld = (m.k + lmg::kTK - 1) / lmg::kTK * lmg::kTK;
+    const int rows = (m.rows + 63) / 64 * 64, ld = (m.k + 31) / 32 * 32;
     std::vector<uint16_t> hi(size_t(rows) * ld, 0), lo(size_t(rows) * ld, 0);
     for (int r = 0; r < m.rows; ++r)
         for (int k = 0; k < m.k; ++k) {
@@ -556,6 +604,7 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
     s->kx = s->px / 2 + 1;
     s->V = (long long)s->nx * s->ny * s->nz;
     s->nw = (long long)s->pz * s->py * s->kx;
+    s->dims = lmg::make_dims(s->nx, s->ny, s->nz);
     s->cap = max_batch;
     const long long two_kx = 2 * s->kx;
     const long long n1 = (long long)s->nz * s->ny * two_kx, n2 = (long long)s->nz * s->py * two_kx;
@@ -592,7 +641,9 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
         (st = salloc(s, (void**)&s->map, size_t(s->V) * max_batch * 4)) ||
         (st = salloc(s, (void**)&s->tgt, size_t(s->V) * max_batch * 4)) ||
         (st = salloc(s, (void**)&s->part, kRedBlocks * 2 * sizeof(double))) ||
-        (st = salloc(s, (void**)&s->red, 8 * sizeof(double))) || (st = salloc(s, (void**)&s->bad, sizeof(int))))
+        (st = salloc(s, (void**)&s->red, 8 * sizeof(double))) || (st = salloc(s, (void**)&s->bad, sizeof(int))) ||
+        (st = salloc(s, (void**)&s->gate_err, sizeof(double))) ||
+        (st = salloc(s, (void**)&s->gate_run, sizeof(int32_t))))
         return fail_create(st);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->w), s->nw * 2, 0u);
@@ -608,9 +659,16 @@ void lmg_surrogate_destroy(lmg_surrogate* s) {
     if (!s) return;
     if (hipSetDevice(s->dev) == hipSuccess) {
         (void)hipStreamSynchronize(s->stream);
+        if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
         for (void* p : s->allocs) (void)hipFree(p);
     }
     delete s;
+}
+
+lmg_status lmg_surrogate_set_graphs(lmg_surrogate* s, int32_t on) {
+    if (!s) return LMG_ERR_ARG;
+    s->use_graphs = on != 0;
+    return LMG_OK;
 }
 
 const char* lmg_surrogate_last_error(const lmg_surrogate* s) {
@@ -797,6 +855,97 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
     return LMG_OK;
 }
 
+namespace {
+
+// Every launch of one gated map, on s->stream (also what a graph captures).
+void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int n, float* d_map,
+                   double tolerance, int gate_sites) {
+    const hipStream_t st = s->stream;
+    // occupancy grid (slot 0 of s->occ)
+    const long long words = (s->V + 3) / 4;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(words)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(s->occ),
+                       words, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(s->bad), 1LL, 0u);
+    if (n > 0)
+        hipLaunchKernelGGL(occ_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, d_site, d_q, n, s->V, s->occ,
+                           s->bad);
+    forward(s, s->w, s->occ, 1, d_map);
+    // exact pair sums at the gate sites
+    const bool gauss = s->flags & LMG_GAUSS;
+    const uint32_t c2m1 = lmg::cutoff_bits(s->p.cutoff, s->flags & LMG_NO_CUTOFF);
+    const float gk = lmg::gauss_k(s->p.gaussian_sigma);
+    if (n > 0)
+        hipLaunchKernelGGL(lmg::to_float4_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->dims, d_site, d_q, n,
+                           s->chg, nullptr);
+    const int S = lmg::split_count(gate_sites, n);
+    const dim3 pgrid((gate_sites + lmg::kBlock - 1) / lmg::kBlock, S);
+    if (gauss)
+        hipLaunchKernelGGL((lmg::pair_kernel<true, false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
+                           s->chg, n, c2m1, gk, s->gate_part, gate_sites);
+    else
+        hipLaunchKernelGGL((lmg::pair_kernel<false, false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
+                           s->chg, n, c2m1, gk, s->gate_part, gate_sites);
+    const int per = lmg::kBlock / lmg::kRedLanes;
+    hipLaunchKernelGGL(lmg::pair_reduce_kernel, dim3((gate_sites + per - 1) / per), dim3(lmg::kBlock), 0, st,
+                       s->gate_part, gate_sites, S, gate_sites, s->p.prefactor, s->gate_exact);
+    const int nblk = std::min(kRedBlocks, (gate_sites + kRedThreads - 1) / kRedThreads);
+    hipLaunchKernelGGL(gate_part_kernel, dim3(nblk), dim3(kRedThreads), 0, st, d_map, s->gate_site, s->gate_exact,
+                       gate_sites, s->part);
+    hipLaunchKernelGGL(gate_final_kernel, dim3(1), dim3(kRedThreads), 0, st, s->part, nblk, tolerance, s->gate_err,
+                       s->gate_run);
+    // exact map over the surrogate's when the gate failed (decided on the device)
+    int tx = 1;
+    while (tx < 64 && tx * lmg::kMapT < s->nx) tx <<= 1;
+    const int ty = lmg::kBlock / tx;
+    const dim3 mgrid((s->nx + tx * lmg::kMapT - 1) / (tx * lmg::kMapT), (s->ny + ty - 1) / ty, s->nz);
+    const float pref = float(s->p.prefactor);
+    if (gauss)
+        hipLaunchKernelGGL(lmg::map_kernel<true>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, n, c2m1, gk, pref,
+                           tx, s->gate_run, d_map);
+    else
+        hipLaunchKernelGGL(lmg::map_kernel<false>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, n, c2m1, gk,
+                           pref, tx, s->gate_run, d_map);
+}
+
+lmg_status gate_buffers(lmg_surrogate* s, int n, int gate_sites, uint64_t gate_seed) {
+    lmg_status st;
+    auto regrow = [&](void** p, size_t bytes) -> lmg_status {
+        if (*p) {
+            (void)hipStreamSynchronize(s->stream);
+            (void)hipFree(*p);
+            s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), *p));
+            *p = nullptr;
+        }
+        ++s->buf_gen;
+        return salloc(s, p, bytes);
+    };
+    if (gate_sites > s->gate_cap) {
+        if ((st = regrow((void**)&s->gate_site, size_t(gate_sites) * 4)) ||
+            (st = regrow((void**)&s->gate_tgt, size_t(gate_sites) * sizeof(float4))) ||
+            (st = regrow((void**)&s->gate_exact, size_t(gate_sites) * 8)) ||
+            (st = regrow((void**)&s->gate_part, size_t(kGateMaxSplit) * gate_sites * sizeof(float2))))
+            return st;
+        s->gate_cap = gate_sites;
+        s->gate_n = 0;
+    }
+    if (n > s->chg_cap) {
+        const int cap = std::max(n, 2 * s->chg_cap);
+        if ((st = regrow((void**)&s->chg, size_t(cap) * sizeof(float4))) != LMG_OK) return st;
+        s->chg_cap = cap;
+    }
+    if (s->gate_n != gate_sites || s->gate_seed != gate_seed) {
+        hipLaunchKernelGGL(gate_site_kernel, dim3((gate_sites + 255) / 256), dim3(256), 0, s->stream, s->gate_site,
+                           gate_sites, s->V, (unsigned long long)gate_seed);
+        hipLaunchKernelGGL(lmg::to_float4_kernel, dim3((gate_sites + 255) / 256), dim3(256), 0, s->stream, s->dims,
+                           s->gate_site, nullptr, gate_sites, s->gate_tgt, nullptr);
+        s->gate_n = gate_sites;
+        s->gate_seed = gate_seed;
+    }
+    return launch_ok(s, "gate set-up");
+}
+
+}  // namespace
+
 lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n, float* d_map,
                                  double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
                                  int32_t* used_exact) {
@@ -805,40 +954,50 @@ lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const 
         return sfail(s, LMG_ERR_ARG, "invalid map arguments");
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
-    if (gate_sites > s->gate_cap) {
-        void* a = nullptr;
-        void* b = nullptr;
-        if ((st = salloc(s, &a, size_t(gate_sites) * 4)) || (st = salloc(s, &b, size_t(gate_sites) * 8))) return st;
-        s->gate_site = static_cast<int32_t*>(a);
-        s->gate_exact = static_cast<double*>(b);
-        s->gate_cap = gate_sites;
+    if ((st = gate_buffers(s, n, gate_sites, gate_seed)) != LMG_OK) return st;
+    if (!s->use_graphs) {
+        enqueue_gated(s, d_site, d_q, n, d_map, tolerance, gate_sites);
+        if ((st = launch_ok(s, "surrogate map")) != LMG_OK) return st;
+    } else {
+        const lmg_surrogate::Key key{d_site, d_q, d_map, n, gate_sites, tolerance, s->buf_gen};
+        if (!s->gexec || !(key == s->gkey)) {
+            if (s->gexec) {
+                SHIP(s, hipStreamSynchronize(s->stream));
+                SHIP(s, hipGraphExecDestroy(s->gexec));
+                s->gexec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            SHIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+            enqueue_gated(s, d_site, d_q, n, d_map, tolerance, gate_sites);
+            const hipError_t cap = hipStreamEndCapture(s->stream, &graph);
+            if (cap != hipSuccess) return sfail(s, LMG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(cap));
+            const hipError_t inst = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (inst != hipSuccess) {
+                s->gexec = nullptr;
+                return sfail(s, LMG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
+            }
+            s->gkey = key;
+        }
+        SHIP(s, hipGraphLaunch(s->gexec, s->stream));
     }
-    if ((st = lmg_surrogate_occupancy_dev(s, d_site, d_q, n, s->occ)) != LMG_OK) return st;
-    forward(s, s->w, s->occ, 1, d_map);
-    hipLaunchKernelGGL(gate_site_kernel, dim3((gate_sites + 255) / 256), dim3(256), 0, s->stream, s->gate_site,
-                       gate_sites, s->V, (unsigned long long)gate_seed);
-    const int pair_flags = s->flags & (LMG_GAUSS | LMG_NO_CUTOFF);
-    if ((st = lmg_pair_potential_dev(s->ctx, d_site, d_q, n, s->gate_site, gate_sites, pair_flags, s->gate_exact)) !=
-        LMG_OK)
-        return sfail(s, st, std::string("gate pair sum: ") + lmg_last_error(s->ctx));
-    hipLaunchKernelGGL(gate_err_kernel, dim3(1), dim3(kRedThreads), 0, s->stream, d_map, s->gate_site, s->gate_exact,
-                       gate_sites, s->red + 6);
-    if ((st = launch_ok(s, "surrogate map")) != LMG_OK) return st;
-    double r[2];
-    int bad = 0;
-    SHIP(s, hipMemcpyAsync(r, s->red + 6, sizeof(r), hipMemcpyDeviceToHost, s->stream));
-    SHIP(s, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (!gate_error && !used_exact) return LMG_OK;
+    return lmg_surrogate_last_gate(s, gate_error, used_exact);
+}
+
+lmg_status lmg_surrogate_last_gate(lmg_surrogate* s, double* gate_error, int32_t* used_exact) {
+    if (!s) return LMG_ERR_ARG;
+    lmg_status st;
+    if ((st = use_dev(s)) != LMG_OK) return st;
+    double e = 0.0;
+    int32_t run = 0, bad = 0;
+    SHIP(s, hipMemcpyAsync(&e, s->gate_err, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipMemcpyAsync(&run, s->gate_run, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipMemcpyAsync(&bad, s->bad, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     SHIP(s, hipStreamSynchronize(s->stream));
     if (bad) return sfail(s, LMG_ERR_ARG, "charge site outside the lattice or |q| > 127");
-    const double e = r[1] > 0 ? std::sqrt(r[0] / r[1]) : (r[0] > 0 ? INFINITY : 0.0);
     if (gate_error) *gate_error = e;
-    const bool fallback = !(e <= tolerance);
-    if (used_exact) *used_exact = fallback ? 1 : 0;
-    if (fallback) {
-        if ((st = lmg_potential_map_dev(s->ctx, d_site, d_q, n, s->flags, d_map)) != LMG_OK)
-            return sfail(s, st, std::string("exact map: ") + lmg_last_error(s->ctx));
-        SHIP(s, hipStreamSynchronize(s->stream));
-    }
+    if (used_exact) *used_exact = run;
     return LMG_OK;
 }
 

@@ -108,13 +108,30 @@ class Surrogate:
         return hist[:int(iterations)]
 
     def map_dev(self, site_ptr: int, q_ptr: int, n: int, map_ptr: int, tolerance: float = 1e-3,
-                gate_sites: int = 4096, gate_seed: int = 1) -> Tuple[float, bool]:
-        """Gated map; returns (sampled relative RMS error, exact fallback used)."""
+                gate_sites: int = 2048, gate_seed: int = 1, wait: bool = True):
+        """Gated map; returns (sampled relative RMS error, exact fallback used),
+        or None with wait=False (asynchronous; see :meth:`last_gate`)."""
+        if not wait:
+            self._chk(_abi.lib().lmg_surrogate_map_dev(self._h, site_ptr, q_ptr, int(n), map_ptr,
+                                                       float(tolerance), int(gate_sites), C.c_uint64(gate_seed),
+                                                       None, None))
+            return None
         err = C.c_double()
         used = C.c_int32()
         self._chk(_abi.lib().lmg_surrogate_map_dev(self._h, site_ptr, q_ptr, int(n), map_ptr, float(tolerance),
                                                    int(gate_sites), C.c_uint64(gate_seed), C.byref(err),
                                                    C.byref(used)))
+        return float(err.value), bool(used.value)
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay the gated map as a recorded HIP graph (default) or launch
+        every kernel each call."""
+        self._chk(_abi.lib().lmg_surrogate_set_graphs(self._h, int(bool(on))))
+
+    def last_gate(self) -> Tuple[float, bool]:
+        err = C.c_double()
+        used = C.c_int32()
+        self._chk(_abi.lib().lmg_surrogate_last_gate(self._h, C.byref(err), C.byref(used)))
         return float(err.value), bool(used.value)
 
     # -- host conveniences -----------------------------------------------------
@@ -144,7 +161,7 @@ class Surrogate:
         g = d_g.download(np.float32, 2 * self.n_weights)
         return (g[0::2] + 1j * g[1::2]).reshape(self.spectrum_shape), loss
 
-    def map(self, sites, charges, tolerance: float = 1e-3, gate_sites: int = 4096,
+    def map(self, sites, charges, tolerance: float = 1e-3, gate_sites: int = 2048,
             gate_seed: int = 1) -> Tuple[np.ndarray, float, bool]:
         s = DeviceBuffer.from_array(np.asarray(sites, np.int32))
         q = DeviceBuffer.from_array(np.asarray(charges, np.int32))

@@ -110,6 +110,36 @@ def test_gate_accepts_ideal_weights_and_rejects_untrained(gpu, oracle):
     assert used and np.array_equal(out, exact)
 
 
+def test_graph_replays_match_eager_launches(gpu):
+    """The gated map replayed from its recorded graph, with other device work
+    and synchronisations between the replays, equals the eagerly launched
+    sequence every time (both for an accepted and a rejected gate)."""
+    dims, cutoff = (24, 20, 8), 6
+    rng = np.random.default_rng(8)
+    V = int(np.prod(dims))
+    ctx, sur = make(gpu, dims, cutoff)
+    sites, q = random_charges(rng, V, 120)
+    d_s, d_q = gpu.DeviceBuffer.from_array(sites), gpu.DeviceBuffer.from_array(q)
+    d_out = gpu.DeviceBuffer(4 * V)
+    other = np.stack([S.occupancy(dims, *random_charges(rng, V, 50))])
+    for weights in (S.ideal_weights(dims, cutoff, PREF), None):
+        sur.set_weights(weights)
+        sur.set_graphs(False)
+        err0, used0 = sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3)
+        eager = d_out.download(np.float32, V)
+        sur.set_graphs(True)
+        for rep in range(4):
+            d_out.upload(np.full(V, np.nan, np.float32))
+            err, used = sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3)
+            assert (err, used) == (err0, used0)
+            assert np.array_equal(d_out.download(np.float32, V), eager), f"replay {rep}"
+            sur.infer(other)                     # eager work between replays
+        sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3, wait=False)
+        ctx.sync()
+        assert sur.last_gate() == (err0, used0)
+        assert np.array_equal(d_out.download(np.float32, V), eager)
+
+
 def solar_snapshots(gpu, sur, dims, cutoff, count, stride, seed):
     p = gpu.SimulationParameters(
         simulation_type="solarcell", grid_x=dims[0], grid_y=dims[1], grid_z=dims[2], coulomb_carriers=True,
@@ -191,7 +221,8 @@ def test_full_size_gain_over_exact_map(gpu):
         ctx.sync()
         return (time.perf_counter() - t) / reps
 
-    t_sur = timed(lambda: sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3))
+    t_sur = timed(lambda: sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 1e-3, wait=False))
+    assert sur.last_gate()[1] is False
     t_exact = timed(lambda: ctx.potential_map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 0))
     print(f"surrogate map {t_sur * 1e3:.3f} ms, exact map {t_exact * 1e3:.3f} ms, gain {t_exact / t_sur:.1f}x")
     assert t_exact / t_sur >= 5.0
