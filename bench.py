@@ -234,6 +234,10 @@ def bench_surrogate(workload, n_snap=64, stride=25, iters=1200, reps=50):
     snapshots of the KMC run on the device labelled by the exact map kernel,
     training on the device, then held-out accuracy and gated-map throughput
     against the exact map on the same charges."""
+    return _bench_surrogate(workload, n_snap, stride, iters, reps)
+
+
+def _bench_surrogate(workload, n_snap, stride, iters, reps):
     import langmuir_amd as L
     p, e, h, traps, pot = workload
     dims = (p.grid_x, p.grid_y, p.grid_z)
@@ -304,6 +308,15 @@ def bench_surrogate(workload, n_snap=64, stride=25, iters=1200, reps=50):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            return json.load(f)["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(workload, budget_s=15.0):
     """FP64 oracle (restated reference CPU path) on the same workload, all host cores."""
     from oracle import pyoracle as O
@@ -349,6 +362,13 @@ def main():
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-iters", type=int, default=1200,
+                    help="training steps of the surrogate (fewer for profiler runs)")
+    ap.add_argument("--surrogate-snapshots", type=int, default=64,
+                    help="KMC snapshots for the surrogate (8 of them held out)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (calibrates FETCH_SIZE/WRITE_SIZE "
+                         "counters against a transfer of known size)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -357,6 +377,14 @@ def main():
         sys.exit(2)
 
     import langmuir_amd as L
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        a, b = L.DeviceBuffer(nbytes), L.DeviceBuffer(nbytes)
+        from langmuir_amd.device import D2D, hip
+        assert hip().hipMemcpy(C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(nbytes), D2D) == 0
+        a.free()
+        b.free()
+        log(f"copy probe: {args.copy_probe} MiB device to device")
     workload = solar_workload() if args.workload == "solar" else transistor_workload()
     p, e, h, traps, pot = workload
     log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(e)} e + {len(h)} h, "
@@ -417,7 +445,9 @@ def main():
                       "event_span_per_step": round(span_ms / args.steps * 1e3, 3)},
         "roofline": {"bound": "valu", "kernel": "de_kernel", "achieved": round(achieved, 4),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": measured_traffic("de_kernel"),
+                     "traffic_source": "profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                       "scaled by a 256 MiB copy probe)",
                      "per_launch": f"11 flops x {terms} in-cutoff terms"},
     }
     if not args.no_map:
@@ -425,7 +455,8 @@ def main():
     if not args.no_batch:
         res["batch"] = bench_batch()
     if not args.no_surrogate and args.workload == "solar":
-        res["surrogate"] = bench_surrogate(workload)
+        res["surrogate"] = bench_surrogate(workload, n_snap=max(16, args.surrogate_snapshots),
+                                           iters=args.surrogate_iters)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(workload, args.cpu_budget)
         res["cpu_baseline"] = cb
