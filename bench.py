@@ -296,7 +296,7 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
         "speedup_vs_exact": round(ms_exact / ms_gated, 2),
         "gated_map_us": round(ms_gated * 1e3, 2), "infer_us": round(ms_infer * 1e3, 2),
         "exact_map_us": round(ms_exact * 1e3, 2),
-        "rel_rms_heldout_max": max(errs), "gate_error": gate_err, "gate_sites": 2048, "used_exact": used,
+        "rel_rms_heldout_max": max(errs), "gate_error": gate_err, "gate_sites": 1024, "used_exact": used,
         "rel_rms_gated_map": full_err,
         "weights_stored": 2 * sur.n_weights, "weights_free": free, "padded": list(sur.padded),
         "snapshots": n_snap, "snapshot_stride_steps": stride, "train_snapshots": n_train,

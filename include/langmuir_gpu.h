@@ -255,7 +255,7 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
                                    int32_t batch, int32_t iterations, double momentum, uint64_t seed,
                                    float* loss_history);
 /* Gated map of a charge list: surrogate map, exact pair sums at gate_sites
- * sampled sites (seeded), gate error = relative RMS error over them; when it
+ * (1..8192) sampled sites (seeded), gate error = relative RMS error over them; when it
  * exceeds tolerance the exact map (lmg_potential_map_if_dev) overwrites d_map.
  * The decision is taken on the device.  With gate_error and used_exact both
  * NULL the call is asynchronous on the context stream (lmg_surrogate_last_gate

@@ -47,22 +47,27 @@ constexpr double kStepFactor = 0.9;
 constexpr int kPrecondSnapshots = 256;   // occupancies averaged into the preconditioner
 constexpr double kPrecondFloor = 1e-3;   // relative floor of the spectral power
 constexpr int kGateMaxSplit = 128;       // charge chunks of the gate's pair sums (split_count's cap)
+constexpr int kMaxGateSites = kRedBlocks * 16;   // gate_reduce_kernel's block sums fit the partial buffer
 
-// Mirror images of spectrum index k = (kz Py + ky) Kx + kx under ky -> -ky and
-// kz -> -kz (the lattice reflections y -> -y, z -> -z; x -> -x is the complex
-// conjugate, which realness of the weights covers).
+// Spectrum layout (one sample): complex values (re, im) at index
+// k = (kx Py + ky) Pz + kz, i.e. rows (kx, ky) of Pz interleaved pairs.
 struct SpecDims {
     int py, pz, kxn;
 };
+
+// Mirror images of k under ky -> -ky and kz -> -kz (the lattice reflections
+// y -> -y, z -> -z; x -> -x is the complex conjugate, which realness of the
+// weights covers).
 __device__ __forceinline__ void spec_images(long long k, SpecDims sd, long long img[4]) {
-    const int kx = int(k % sd.kxn);
-    const long long r = k / sd.kxn;
-    const int ky = int(r % sd.py), kz = int(r / sd.py);
+    const int kz = int(k % sd.pz);
+    const long long r = k / sd.pz;
+    const int ky = int(r % sd.py);
+    const long long kx = r / sd.py;
     const int my = ky ? sd.py - ky : 0, mz = kz ? sd.pz - kz : 0;
     img[0] = k;
-    img[1] = ((long long)kz * sd.py + my) * sd.kxn + kx;
-    img[2] = ((long long)mz * sd.py + ky) * sd.kxn + kx;
-    img[3] = ((long long)mz * sd.py + my) * sd.kxn + kx;
+    img[1] = (kx * sd.py + my) * sd.pz + kz;
+    img[2] = (kx * sd.py + ky) * sd.pz + mz;
+    img[3] = (kx * sd.py + my) * sd.pz + mz;
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -104,16 +109,6 @@ __global__ void occ_scatter_kernel(const int32_t* __restrict__ site, const int32
         const unsigned int nv = (assumed & ~(0xffu << sh)) | (unsigned(uint8_t(int8_t(b))) << sh);
         old = atomicCAS(w, assumed, nv);
     } while (old != assumed);
-}
-
-// y[b][k] = w[k] x[b][k] on interleaved complex values
-__global__ void spec_mult_kernel(const float2* __restrict__ w, const float2* __restrict__ x, float2* __restrict__ y,
-                                 long long nw, int nb) {
-    const long long n = nw * nb;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const float2 a = w[i % nw], b = x[i];
-        y[i] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-    }
 }
 
 // map := 2 (map - target); part[2 blk] = sum (map - target)^2, part[2 blk + 1] =
@@ -172,6 +167,20 @@ __global__ void grad_kernel(const float2* __restrict__ g, const float2* __restri
     }
 }
 
+// dst in the weight interchange layout ([kz][ky][kx] interleaved re, im) =
+// scale / den[0] * src (spectrum layout)
+__global__ void to_interchange_kernel(const float2* __restrict__ src, float2* __restrict__ dst, SpecDims sd,
+                                      long long nw, const double* __restrict__ den) {
+    const float sc = den ? float(1.0 / den[0]) : 1.f;
+    for (long long a = blockIdx.x * (long long)blockDim.x + threadIdx.x; a < nw; a += (long long)gridDim.x * blockDim.x) {
+        const int kx = int(a % sd.kxn);
+        const long long t = a / sd.kxn;
+        const int ky = int(t % sd.py), kz = int(t / sd.py);
+        const float2 v = src[((long long)kx * sd.py + ky) * sd.pz + kz];
+        dst[a] = make_float2(sc * v.x, sc * v.y);
+    }
+}
+
 // out = src * scale / den[0] (den NULL: 1)
 __global__ void scale_kernel(float* __restrict__ out, const float* __restrict__ src, long long n, float scale,
                              const double* __restrict__ den) {
@@ -215,14 +224,18 @@ __global__ void precond_kernel(const double* __restrict__ h, long long nw, SpecD
     }
 }
 
+__device__ __forceinline__ float sym_re(const float2* __restrict__ g, long long k, SpecDims sd) {
+    long long img[4];
+    spec_images(k, sd, img);
+    return 0.25f * ((g[img[0]].x + g[img[1]].x) + (g[img[2]].x + g[img[3]].x));
+}
+
 // out[k] = pk[k] * (Re g averaged over the mirror images), 0): the projection
 // onto real weights even in ky and kz, preconditioned (pk NULL: 1)
 __global__ void sym_kernel(const float2* __restrict__ g, float2* __restrict__ out, long long nw, SpecDims sd,
                            const float* __restrict__ pk) {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nw; k += (long long)gridDim.x * blockDim.x) {
-        long long img[4];
-        spec_images(k, sd, img);
-        float v = 0.25f * ((g[img[0]].x + g[img[1]].x) + (g[img[2]].x + g[img[3]].x));
+        float v = sym_re(g, k, sd);
         if (pk) v *= pk[k];
         out[k] = make_float2(v, 0.f);
     }
@@ -235,10 +248,7 @@ __global__ void sgd_sym_kernel(float2* __restrict__ w, float* __restrict__ mom, 
                                float lr, float mu, float* __restrict__ loss) {
     const float inv = float(1.0 / red[1]);
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nw; k += (long long)gridDim.x * blockDim.x) {
-        long long img[4];
-        spec_images(k, sd, img);
-        const float g = 0.25f * ((grad[img[0]].x + grad[img[1]].x) + (grad[img[2]].x + grad[img[3]].x));
-        const float m = mu * mom[k] + pk[k] * g * inv;
+        const float m = mu * mom[k] + pk[k] * sym_re(grad, k, sd) * inv;
         mom[k] = m;
         w[k] = make_float2(w[k].x - lr * m, 0.f);
     }
@@ -257,35 +267,88 @@ __global__ void gate_site_kernel(int32_t* site, int n, long long V, unsigned lon
     if (i < n) site[i] = int32_t(splitmix64(seed * 0x100000001b3ull + (unsigned long long)i) % (unsigned long long)V);
 }
 
-// part[2 blk] = sum (map[site] - exact)^2, part[2 blk + 1] = sum exact^2 over
-// the block's grid-stride share of the gate sites
-__global__ void gate_part_kernel(const float* __restrict__ map, const int32_t* __restrict__ site,
-                                 const double* __restrict__ exact, int n, double* __restrict__ part) {
-    __shared__ double sh[kRedThreads];
-    double a = 0.0, b = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double e = exact[i];
-        const double d = double(map[site[i]]) - e;
-        a += d * d;
-        b += e * e;
-    }
-    a = block_sum(a, sh);
-    b = block_sum(b, sh);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = a;
-        part[2 * blockIdx.x + 1] = b;
-    }
+// Occupancy grid words and the bad-input flag cleared in one launch.
+__global__ void occ_clear_kernel(uint32_t* __restrict__ occ, long long words, int* __restrict__ bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bad = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < words; i += (long long)gridDim.x * blockDim.x)
+        occ[i] = 0u;
 }
 
-// err = sqrt(sum d^2 / sum e^2) over the gate sites; *run = err > tol (the
-// exact map must replace the surrogate's)
-__global__ void gate_final_kernel(const double* __restrict__ part, int nblk, double tol, double* __restrict__ err,
-                                  int32_t* __restrict__ run) {
-    __shared__ double sh[kRedThreads];
+// occ_scatter_kernel plus the (x, y, z, q) coordinates of every charge for the
+// exact pair sums (the work of to_float4_kernel), in one pass over the charges.
+__global__ void occ_scatter_f4_kernel(const int32_t* __restrict__ site, const int32_t* __restrict__ q, int n,
+                                      lmg::Dims d, int8_t* occ, int* bad, float4* __restrict__ chg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = site[i];
+    const int qi = q[i];
+    if (s < 0 || s >= d.V || qi < -127 || qi > 127) {
+        atomicOr(bad, 1);
+        chg[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    int x, y, z;
+    lmg::decode(d, s, x, y, z);
+    chg[i] = make_float4(float(x), float(y), float(z), float(qi));
+    unsigned int* w = reinterpret_cast<unsigned int*>(occ + (s & ~3));
+    const int sh = (s & 3) * 8;
+    unsigned int old = *w, assumed;
+    do {
+        assumed = old;
+        const int b = int(int8_t((assumed >> sh) & 0xffu)) + qi;
+        const unsigned int nv = (assumed & ~(0xffu << sh)) | (unsigned(uint8_t(int8_t(b))) << sh);
+        old = atomicCAS(w, assumed, nv);
+    } while (old != assumed);
+}
+
+// The gate in one launch: the exact potential at each gate site (the S chunk
+// partials of pair_kernel summed in FP64 like pair_reduce_kernel, 16 lanes per
+// site), its squared difference from the map, block sums, and in the last block
+// to finish the total over blocks in block order: err = sqrt(sum d^2 / sum e^2)
+// and *run = err > tol.  The finishing counter returns to zero for the next call.
+constexpr int kGateLanes = 16;
+__global__ __launch_bounds__(256) void gate_reduce_kernel(const float2* __restrict__ part, int nt, int S, double scale,
+                                                          const float* __restrict__ map,
+                                                          const int32_t* __restrict__ site, double* __restrict__ bpart,
+                                                          unsigned int* __restrict__ counter, double tol,
+                                                          double* __restrict__ err, int32_t* __restrict__ run) {
+    __shared__ double sh[256];
+    __shared__ bool last;
+    const int j = threadIdx.x % kGateLanes;
+    const int t = blockIdx.x * (256 / kGateLanes) + threadIdx.x / kGateLanes;
+    double v = 0.0;
+    if (t < nt)
+        for (int c = j; c < S; c += kGateLanes) {
+            const float2 p = part[(long long)c * nt + t];
+            v += double(p.x) + double(p.y);
+        }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    double dd = 0.0, ee = 0.0;
+    if (j == 0 && t < nt) {
+        double a = 0.0;
+        for (int i = 0; i < kGateLanes; ++i) a += sh[threadIdx.x + i];
+        const double e = scale * a, d = double(map[site[t]]) - e;
+        dd = d * d;
+        ee = e * e;
+    }
+    __syncthreads();
+    dd = block_sum(dd, sh);
+    ee = block_sum(ee, sh);
+    if (threadIdx.x == 0) {
+        bpart[2 * blockIdx.x] = dd;
+        bpart[2 * blockIdx.x + 1] = ee;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        a += part[2 * i];
-        b += part[2 * i + 1];
+    const volatile double* vp = bpart;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+        a += vp[2 * i];
+        b += vp[2 * i + 1];
     }
     a = block_sum(a, sh);
     b = block_sum(b, sh);
@@ -293,6 +356,7 @@ __global__ void gate_final_kernel(const double* __restrict__ part, int nblk, dou
         const double e = b > 0 ? sqrt(a / b) : (a > 0 ? INFINITY : 0.0);
         *err = e;
         *run = (e <= tol) ? 0 : 1;
+        *counter = 0u;
     }
 }
 
@@ -310,7 +374,8 @@ struct lmg_surrogate {
     int nx = 0, ny = 0, nz = 0, px = 0, py = 0, pz = 0, kx = 0;
     long long V = 0, nw = 0;       // lattice sites; complex weights
     int cap = 0;                   // batch capacity
-    lmg::ConstOperand c_fx{}, c_fy{}, c_fz{}, c_gz{}, c_gy{}, c_hx{}, c_hxt{};   // DFT matrices
+    // DFT matrices: forward x, y, z; inverse z, y, x; adjoint of the inverse x
+    lmg::ConstOperand c_x{}, c_y{}, c_z{}, c_zi{}, c_yi{}, c_xi{}, c_xadj{};
     float *w = nullptr, *mom = nullptr, *grad = nullptr, *vec = nullptr;
     float* pk = nullptr;           // preconditioner (nw)
     double* hpow = nullptr;        // spectral power accumulator (nw)
@@ -326,13 +391,13 @@ struct lmg_surrogate {
     lmg::Dims dims{};
     int32_t* gate_site = nullptr;  // sampled sites
     float4* gate_tgt = nullptr;    // their coordinates
-    double* gate_exact = nullptr;  // exact potential there
     float2* gate_part = nullptr;   // pair-sum partials, kGateMaxSplit x gate_cap
     int gate_cap = 0;
     int gate_n = 0;                // sites currently in gate_site, and their seed
     uint64_t gate_seed = 0;
     float4* chg = nullptr;         // charges as (x, y, z, q)
     int chg_cap = 0;
+    unsigned int* gate_counter = nullptr;   // blocks finished in gate_reduce_kernel (0 between calls)
     double* gate_err = nullptr;    // device: latest gate error
     int32_t* gate_run = nullptr;   // device: latest decision (1 = exact map used)
     // the gated map as a replayable graph, valid while its key holds
@@ -372,70 +437,84 @@ lmg_status salloc(lmg_surrogate* s, void** p, size_t bytes) {
     return LMG_OK;
 }
 
-int padded(int n, int cutoff, bool none) {
+// At least N + c - 1 sites (2N - 1 without a cutoff), so the circular
+// convolution equals the aperiodic one; x rounded up to 6 mod 8 (Px / 2 + 1 a
+// multiple of 4), y and z to a multiple of 4, for the 16-byte stores of the
+// transposed stage outputs.
+int padded(int n, int cutoff, bool none, bool x_axis) {
     if (n <= 1) return 1;
-    long long m = none ? 2LL * n - 1 : std::min<long long>(2LL * n - 1, (long long)n + cutoff - 1);
-    return int(m + (m & 1));
+    const long long m = none ? 2LL * n - 1 : std::min<long long>(2LL * n - 1, (long long)n + cutoff - 1);
+    return int(x_axis ? m + ((6 - m) % 8 + 8) % 8 : m + ((-m) % 4 + 4) % 4);
 }
 
-// C (M x N) = D (M x K) * Bc: data rows with k contiguous, constant Bc stored [n][k]
+lmg::OutLayout row_major(int ldc) { return lmg::OutLayout{0, ldc, 1, 0, 1, 0, 0, 0}; }
+
+// C[row][col] stored at (row / inner) s_ro + row % inner + (col % cmod) s_cm + (col / cmod) s_cd
+lmg::OutLayout transposed(int inner, int s_ro, int cmod, int s_cm, int s_cd, long long s_batch) {
+    const bool v4 = inner % 4 == 0 && s_ro % 4 == 0 && s_cm % 4 == 0 && s_cd % 4 == 0 && s_batch % 4 == 0;
+    return lmg::OutLayout{1, 0, inner, s_ro, cmod, s_cm, s_cd, v4 ? 1 : 0};
+}
+
+// C[b] (M x N) = D[b] (M x K, k contiguous) * Bc, Bc stored transposed
 template <typename TD>
-void gemm_data_a(hipStream_t st, const TD* d, int ldd, const lmg::ConstOperand& c, float* out, int ldc, int M, int N,
-                 int K) {
-    lmg::GemmArgs g{d, c, out, M, N, K, K, ldd, ldc, 0, 0, 1.f};
-    dim3 grid((N + lmg::kTN - 1) / lmg::kTN, (M + lmg::kTM - 1) / lmg::kTM, 1);
-    hipLaunchKernelGGL((lmg::gemm_bf16x3_kernel<0, TD, false>), grid, dim3(256), 0, st, g);
+void gemm(hipStream_t st, const TD* d, int ldd, long long sd, const lmg::ConstOperand& c, float* out, long long sc,
+          int M, int N, int K, int batch, const lmg::OutLayout& o, const float* w = nullptr, int ldw = 0) {
+    lmg::GemmArgs g{d, w, ldw, c, out, M, N, K, ldd, sd, sc, 1.f, o};
+    dim3 grid((N + lmg::kTN - 1) / lmg::kTN, (M + lmg::kTM - 1) / lmg::kTM, batch);
+    if (w)
+        hipLaunchKernelGGL((lmg::gemm_bf16x3_kernel<TD, true>), grid, dim3(256), 0, st, g);
+    else
+        hipLaunchKernelGGL((lmg::gemm_bf16x3_kernel<TD, false>), grid, dim3(256), 0, st, g);
 }
 
-// C[b] (M x N) = Ac (M x 2 Kb) * [[D[b]], [i D[b]]]: complex left-multiplication
-// of interleaved complex columns, constant Ac = [Re F | Im F] stored [m][k]
-void gemm_cplx_left(hipStream_t st, const lmg::ConstOperand& c, const float* d, int ldd, long long sd, float* out,
-                    int ldc, long long sc, int M, int N, int Kb, int batch) {
-    lmg::GemmArgs g{d, c, out, M, N, 2 * Kb, Kb, ldd, ldc, sd, sc, 1.f};
-    dim3 grid((N + lmg::kTN - 1) / lmg::kTN, (M + lmg::kTM - 1) / lmg::kTM, batch);
-    hipLaunchKernelGGL((lmg::gemm_bf16x3_kernel<1, float, true>), grid, dim3(256), 0, st, g);
-}
+// Stage buffers per sample (split complex: a row holds [Re ... | Im ...]):
+//   s1: T1 [kx][z][ri][y]  (x-stage out, y-stage in), later T6 [z][y][ri][kx]
+//   s2: T2 [kx][ky][ri][z] (y-stage out, z-stage in), later T5 [z][kx][ri][ky]
+//   spectrum [kx][ky][kz] complex (re, im)
+long long n1_of(const lmg_surrogate* s) { return (long long)s->kx * s->nz * 2 * s->ny; }
+long long n2_of(const lmg_surrogate* s) { return (long long)s->kx * s->py * 2 * s->nz; }
 
 // rfft3 of nb occupancy grids (int8) or real grids (float) into spec; the
-// x-stage matrix is the forward DFT for the forward pass and the transpose of
-// the inverse x-stage for the adjoint pass.
+// x-stage matrix is the forward DFT for the forward pass and the adjoint of the
+// inverse x-stage for the backward pass.
 template <typename TA>
 void transform_in(lmg_surrogate* s, const TA* a, const lmg::ConstOperand& xmat, int nb, float* spec) {
-    const int two_kx = 2 * s->kx;
-    const long long row_y = (long long)s->py * two_kx;
-    // x: [b z y][x] -> [b z y][kx]
-    gemm_data_a<TA>(s->stream, a, s->nx, xmat, s->s1, two_kx, nb * s->nz * s->ny, two_kx, s->nx);
-    // y: per (b, z), [y][kx] -> [ky][kx]
-    gemm_cplx_left(s->stream, s->c_fy, s->s1, two_kx, (long long)s->ny * two_kx, s->s2, two_kx, row_y, s->py, two_kx,
-                   s->ny, nb * s->nz);
-    // z: per b, [z][ky kx] -> [kz][ky kx]
-    gemm_cplx_left(s->stream, s->c_fz, s->s2, int(row_y), s->nz * row_y, spec, int(row_y), 2 * s->nw, s->pz,
-                   int(row_y), s->nz, nb);
+    const int nx = s->nx, ny = s->ny, nz = s->nz, py = s->py, pz = s->pz, kx = s->kx;
+    const long long n1 = n1_of(s), n2 = n2_of(s);
+    // x: rows (z, y), k = x -> T1 [kx][z][ri][y]
+    gemm<TA>(s->stream, a, nx, s->V, xmat, s->s1, n1, nz * ny, 2 * kx, nx, nb,
+             transposed(ny, 2 * ny, kx, nz * 2 * ny, ny, n1));
+    // y: rows (kx, z), k = (ri, y) -> T2 [kx][ky][ri][z]
+    gemm<float>(s->stream, s->s1, 2 * ny, n1, s->c_y, s->s2, n2, kx * nz, 2 * py, 2 * ny, nb,
+                transposed(nz, py * 2 * nz, py, 2 * nz, nz, n2));
+    // z: rows (kx, ky), k = (ri, z) -> spectrum [kx][ky][kz] (re, im)
+    gemm<float>(s->stream, s->s2, 2 * nz, n2, s->c_z, spec, 2 * s->nw, kx * py, 2 * pz, 2 * nz, nb, row_major(2 * pz));
 }
 
-// inverse: spectrum y3 [b][kz][ky][kx] -> nb maps (window of the padded grid)
-void transform_out(lmg_surrogate* s, int nb, float* out) {
-    const int two_kx = 2 * s->kx;
-    const long long row_y = (long long)s->py * two_kx;
-    gemm_cplx_left(s->stream, s->c_gz, s->y3, int(row_y), 2 * s->nw, s->s2, int(row_y), s->nz * row_y, s->nz,
-                   int(row_y), s->pz, nb);
-    gemm_cplx_left(s->stream, s->c_gy, s->s2, two_kx, row_y, s->s1, two_kx, (long long)s->ny * two_kx, s->ny, two_kx,
-                   s->py, nb * s->nz);
-    gemm_data_a<float>(s->stream, s->s1, two_kx, s->c_hx, out, s->nx, nb * s->nz * s->ny, s->nx, two_kx);
+// inverse: nb spectra, each multiplied by the weights w on load -> nb maps (the
+// window of the padded grid)
+void transform_out(lmg_surrogate* s, const float* spec, const float* w, int nb, float* out) {
+    const int nx = s->nx, ny = s->ny, nz = s->nz, py = s->py, pz = s->pz, kx = s->kx;
+    const long long n1 = n1_of(s), n2 = n2_of(s);
+    // inverse z of W . X: rows (kx, ky), k = (kz, re/im) -> T5 [z][kx][ri][ky]
+    gemm<float>(s->stream, spec, 2 * pz, 2 * s->nw, s->c_zi, s->s2, n2, kx * py, 2 * nz, 2 * pz, nb,
+                transposed(py, 2 * py, nz, kx * 2 * py, py, n2), w, 2 * pz);
+    // inverse y: rows (z, kx), k = (ri, ky) -> T6 [z][y][ri][kx]
+    gemm<float>(s->stream, s->s2, 2 * py, n2, s->c_yi, s->s1, n1, nz * kx, 2 * ny, 2 * py, nb,
+                transposed(kx, ny * 2 * kx, ny, 2 * kx, kx, n1));
+    // inverse x: rows (z, y), k = (ri, kx) -> map [z][y][x]
+    gemm<float>(s->stream, s->s1, 2 * kx, n1, s->c_xi, out, s->V, nz * ny, nx, 2 * kx, nb, row_major(nx));
 }
 
 void forward(lmg_surrogate* s, const float* w, const int8_t* occ, int nb, float* out) {
-    transform_in<int8_t>(s, occ, s->c_fx, nb, s->s3);
-    hipLaunchKernelGGL(spec_mult_kernel, dim3(grid_for(s->nw * nb)), dim3(256), 0, s->stream,
-                       reinterpret_cast<const float2*>(w), reinterpret_cast<const float2*>(s->s3),
-                       reinterpret_cast<float2*>(s->y3), s->nw, nb);
-    transform_out(s, nb, out);
+    transform_in<int8_t>(s, occ, s->c_x, nb, s->s3);
+    transform_out(s, s->s3, w, nb, out);
 }
 
 // gradient of sum_b |map_b - t_b|^2 (unnormalised) from g = 2 (map - t) held in
 // s->map, using the spectra s->s3 of the preceding forward pass
 void backward(lmg_surrogate* s, int nb) {
-    transform_in<float>(s, s->map, s->c_hxt, nb, s->y3);
+    transform_in<float>(s, s->map, s->c_xadj, nb, s->y3);
     hipLaunchKernelGGL(grad_kernel, dim3(grid_for(s->nw)), dim3(256), 0, s->stream,
                        reinterpret_cast<const float2*>(s->y3), reinterpret_cast<const float2*>(s->s3), s->nw, nb,
                        reinterpret_cast<float2*>(s->grad));
@@ -477,59 +556,58 @@ struct HostMat {
     double& at(int r, int kk) { return v[size_t(r) * k + kk]; }
 };
 
-// The DFT matrices of the six stages (angles reduced exactly in integers):
-//   fx  [c][x]        forward x (real -> interleaved half spectrum), Bc^T
-//   fy  [ky][y | y]   [cos | -sin] of e^{-2 pi i ky y / Py}; fz likewise
-//   gz  [z][kz | kz]  [cos | +sin] of e^{+2 pi i kz z / Pz}; gy likewise
-//   hx  [x][c]        inverse x (half spectrum -> real, Hermitian weights c_k,
-//                     1 / (Px Py Pz) folded in), Bc^T
-//   hxt [c][x]        transpose of the inverse x-stage (its adjoint), Bc^T
+// The DFT matrices, each stored as Bc^T [output column][k] (angles reduced
+// exactly in integers, theta = 2 pi (o i mod P) / P):
+//   x    [ri kx][x]          forward x: Re = cos, Im = -sin (real input)
+//   y, z [ri ky][ri' y]      complex e^{-i theta} acting on split-complex rows
+//                            (z writes interleaved (re, im) pairs: the spectrum)
+//   zi, yi                   complex e^{+i theta}, spatial outputs (zi reads
+//                            the interleaved spectrum)
+//   xi   [x][ri kx]          inverse x: c_k (cos, -sin) / (Px Py Pz), c_k = 1 at
+//                            kx = 0 and Px / 2, else 2 (Hermitian half spectrum)
+//   xadj [ri kx][x]          transpose of xi (its adjoint)
 void build_matrices(const lmg_surrogate* s, std::vector<HostMat>& out) {
     const double two_pi = 2.0 * M_PI;
     auto ang = [&](long long k, long long x, int p) { return two_pi * double((k * x) % p) / double(p); };
     const int kx = s->kx, nx = s->nx, ny = s->ny, nz = s->nz, px = s->px, py = s->py, pz = s->pz;
     const double inv_total = 1.0 / (double(px) * py * pz);
-    HostMat fx(2 * kx, nx), hx(nx, 2 * kx), hxt(2 * kx, nx);
+    HostMat fx(2 * kx, nx), xi(nx, 2 * kx), xadj(2 * kx, nx);
     for (int x = 0; x < nx; ++x)
         for (int k = 0; k < kx; ++k) {
             const double a = ang(k, x, px);
             const double ck = (k == 0 || (px % 2 == 0 && k == px / 2)) ? 1.0 : 2.0;
-            fx.at(2 * k, x) = std::cos(a);
-            fx.at(2 * k + 1, x) = -std::sin(a);
+            fx.at(k, x) = std::cos(a);
+            fx.at(kx + k, x) = -std::sin(a);
             const double hc = ck * std::cos(a) * inv_total, hs = -ck * std::sin(a) * inv_total;
-            hx.at(x, 2 * k) = hc;
-            hx.at(x, 2 * k + 1) = hs;
-            hxt.at(2 * k, x) = hc;
-            hxt.at(2 * k + 1, x) = hs;
+            xi.at(x, k) = hc;
+            xi.at(x, kx + k) = hs;
+            xadj.at(k, x) = hc;
+            xadj.at(kx + k, x) = hs;
         }
-    auto fwd = [&](int p, int n) {   // p x [cos | -sin] (n each)
-        HostMat m(p, 2 * n);
-        for (int k = 0; k < p; ++k)
-            for (int x = 0; x < n; ++x) {
-                const double a = ang(k, x, p);
-                m.at(k, x) = std::cos(a);
-                m.at(k, n + x) = -std::sin(a);
-            }
-        return m;
-    };
-    auto inv = [&](int n, int p) {   // n x [cos | +sin] (p each)
-        HostMat m(n, 2 * p);
-        for (int x = 0; x < n; ++x)
-            for (int k = 0; k < p; ++k) {
-                const double a = ang(k, x, p);
-                m.at(x, k) = std::cos(a);
-                m.at(x, p + k) = std::sin(a);
+    // complex block acting on split [Re i | Im i] or interleaved (re, im) pairs
+    // of the input, giving split or interleaved outputs, o < no, i < ni
+    auto block = [&](int no, int ni, int p, double sign, bool inter_in, bool inter_out) {
+        HostMat m(2 * no, 2 * ni);
+        auto io = [&](int ri, int o) { return inter_out ? 2 * o + ri : ri * no + o; };
+        auto ii = [&](int ri, int i) { return inter_in ? 2 * i + ri : ri * ni + i; };
+        for (int o = 0; o < no; ++o)
+            for (int i = 0; i < ni; ++i) {
+                const double a = ang(o, i, p), fr = std::cos(a), fi = sign * std::sin(a);
+                m.at(io(0, o), ii(0, i)) = fr;    // Re out <- Re in
+                m.at(io(0, o), ii(1, i)) = -fi;   // Re out <- Im in
+                m.at(io(1, o), ii(0, i)) = fi;    // Im out <- Re in
+                m.at(io(1, o), ii(1, i)) = fr;    // Im out <- Im in
             }
         return m;
     };
     out.clear();
     out.push_back(fx);
-    out.push_back(fwd(py, ny));
-    out.push_back(fwd(pz, nz));
-    out.push_back(inv(nz, pz));
-    out.push_back(inv(ny, py));
-    out.push_back(hx);
-    out.push_back(hxt);
+    out.push_back(block(py, ny, py, -1.0, false, false));
+    out.push_back(block(pz, nz, pz, -1.0, false, true));    // into the interleaved spectrum
+    out.push_back(block(nz, pz, pz, +1.0, true, false));    // out of it
+    out.push_back(block(ny, py, py, +1.0, false, false));
+    out.push_back(xi);
+    out.push_back(xadj);
 }
 
 uint16_t host_bf16(double x) {   // round to nearest even, from the FP32 value
@@ -567,6 +645,24 @@ lmg_status upload_const(lmg_surrogate* s, const HostMat& m, lmg::ConstOperand* c
     return LMG_OK;
 }
 
+// Weights between the interchange layout of the C-ABI ([kz][ky][kx] interleaved
+// re, im) and the spectrum layout ([kx][ky][kz] interleaved re, im).
+void interchange(const lmg_surrogate* s, float* spectrum, float* api, bool to_spectrum) {
+    for (int kz = 0; kz < s->pz; ++kz)
+        for (int ky = 0; ky < s->py; ++ky)
+            for (int kx = 0; kx < s->kx; ++kx) {
+                const size_t a = ((size_t(kz) * s->py + ky) * s->kx + kx) * 2;
+                const size_t r = ((size_t(kx) * s->py + ky) * s->pz + kz) * 2;
+                if (to_spectrum) {
+                    spectrum[r] = api[a];
+                    spectrum[r + 1] = api[a + 1];
+                } else {
+                    api[a] = spectrum[r];
+                    api[a + 1] = spectrum[r + 1];
+                }
+            }
+}
+
 uint64_t host_splitmix(uint64_t& x) {
     uint64_t z = (x += 0x9e3779b97f4a7c15ull);
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -598,19 +694,18 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
     s->ny = p.grid_y;
     s->nz = p.grid_z;
     const bool none = flags & LMG_NO_CUTOFF;
-    s->px = padded(s->nx, p.cutoff, none);
-    s->py = padded(s->ny, p.cutoff, none);
-    s->pz = padded(s->nz, p.cutoff, none);
+    s->px = padded(s->nx, p.cutoff, none, true);
+    s->py = padded(s->ny, p.cutoff, none, false);
+    s->pz = padded(s->nz, p.cutoff, none, false);
     s->kx = s->px / 2 + 1;
     s->V = (long long)s->nx * s->ny * s->nz;
     s->nw = (long long)s->pz * s->py * s->kx;
     s->dims = lmg::make_dims(s->nx, s->ny, s->nz);
     s->cap = max_batch;
-    const long long two_kx = 2 * s->kx;
-    const long long n1 = (long long)s->nz * s->ny * two_kx, n2 = (long long)s->nz * s->py * two_kx;
-    const long long row_tiles = ((long long)max_batch * s->nz * s->ny + lmg::kTM - 1) / lmg::kTM;
-    if (s->nw * 2 * max_batch > (1LL << 31) || s->V * max_batch > (1LL << 31) || row_tiles > 65535 ||
-        (long long)max_batch * s->nz > 65535) {
+    const long long n1 = n1_of(s), n2 = n2_of(s);
+    const long long max_rows = std::max({(long long)s->nz * s->ny, (long long)s->kx * s->nz, (long long)s->kx * s->py});
+    if (s->nw * 2 * max_batch > (1LL << 31) || s->V * max_batch > (1LL << 31) ||
+        (max_rows + lmg::kTM - 1) / lmg::kTM > 65535 || max_batch > 65535) {
         delete s;
         return sfail(nullptr, LMG_ERR_CAPACITY, "lattice x max_batch too large for one surrogate");
     }
@@ -624,7 +719,7 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
     {
         std::vector<HostMat> mats;
         build_matrices(s, mats);
-        lmg::ConstOperand* dst[7] = {&s->c_fx, &s->c_fy, &s->c_fz, &s->c_gz, &s->c_gy, &s->c_hx, &s->c_hxt};
+        lmg::ConstOperand* dst[7] = {&s->c_x, &s->c_y, &s->c_z, &s->c_zi, &s->c_yi, &s->c_xi, &s->c_xadj};
         for (int i = 0; i < 7; ++i)
             if ((st = upload_const(s, mats[i], dst[i])) != LMG_OK) return fail_create(st);
     }
@@ -643,12 +738,14 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
         (st = salloc(s, (void**)&s->part, kRedBlocks * 2 * sizeof(double))) ||
         (st = salloc(s, (void**)&s->red, 8 * sizeof(double))) || (st = salloc(s, (void**)&s->bad, sizeof(int))) ||
         (st = salloc(s, (void**)&s->gate_err, sizeof(double))) ||
+        (st = salloc(s, (void**)&s->gate_counter, sizeof(unsigned int))) ||
         (st = salloc(s, (void**)&s->gate_run, sizeof(int32_t))))
         return fail_create(st);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->w), s->nw * 2, 0u);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->mom), s->nw * 2, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s->stream, s->gate_counter, 1LL, 0u);
     if ((st = launch_ok(s, "surrogate init")) != LMG_OK) return fail_create(st);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return fail_create(sfail(s, LMG_ERR_HIP, "init sync"));
     *out = s;
@@ -689,8 +786,10 @@ lmg_status lmg_surrogate_get_weights(lmg_surrogate* s, float* w) {
     if (!w) return sfail(s, LMG_ERR_ARG, "weights pointer is NULL");
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
-    SHIP(s, hipMemcpyAsync(w, s->w, size_t(s->nw) * 2 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    std::vector<float> in(size_t(s->nw) * 2);
+    SHIP(s, hipMemcpyAsync(in.data(), s->w, in.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     SHIP(s, hipStreamSynchronize(s->stream));
+    interchange(s, in.data(), w, false);
     return LMG_OK;
 }
 
@@ -699,7 +798,10 @@ lmg_status lmg_surrogate_set_weights(lmg_surrogate* s, const float* w) {
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
     if (w) {
-        SHIP(s, hipMemcpyAsync(s->w, w, size_t(s->nw) * 2 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        std::vector<float> in(size_t(s->nw) * 2);
+        interchange(s, in.data(), const_cast<float*>(w), true);
+        SHIP(s, hipMemcpyAsync(s->w, in.data(), in.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        SHIP(s, hipStreamSynchronize(s->stream));
     } else {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                            reinterpret_cast<uint32_t*>(s->w), s->nw * 2, 0u);
@@ -745,8 +847,10 @@ lmg_status lmg_surrogate_gradient_dev(lmg_surrogate* s, const int8_t* d_occ, con
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
     loss_grad(s, s->w, d_occ, d_target, batch);
-    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream, d_grad, s->grad, s->nw * 2,
-                       1.f, s->red + 1);
+    const SpecDims sd{s->py, s->pz, s->kx};
+    hipLaunchKernelGGL(to_interchange_kernel, dim3(grid_for(s->nw)), dim3(256), 0, s->stream,
+                       reinterpret_cast<const float2*>(s->grad), reinterpret_cast<float2*>(d_grad), sd, s->nw,
+                       s->red + 1);
     if ((st = launch_ok(s, "gradient")) != LMG_OK) return st;
     double r[2];
     SHIP(s, hipMemcpyAsync(r, s->red, sizeof(r), hipMemcpyDeviceToHost, s->stream));
@@ -800,7 +904,7 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
         const int nb = std::min(s->cap, n_pre - b0);
         SHIP(s, hipMemcpyAsync(s->occ, d_occ + (long long)b0 * s->V, size_t(s->V) * nb, hipMemcpyDeviceToDevice,
                                s->stream));
-        transform_in<int8_t>(s, s->occ, s->c_fx, nb, s->s3);
+        transform_in<int8_t>(s, s->occ, s->c_x, nb, s->s3);
         hipLaunchKernelGGL(power_accum_kernel, gw, b256, 0, s->stream, reinterpret_cast<const float2*>(s->s3), nw, nb,
                            s->hpow);
     }
@@ -857,26 +961,22 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
 
 namespace {
 
-// Every launch of one gated map, on s->stream (also what a graph captures).
+// Every launch of one gated map, on s->stream (also what a graph captures):
+// occupancy and charge coordinates, the forward pass, the exact pair sums at the
+// gate sites, the gate, and the exact map when the gate failed.
 void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int n, float* d_map,
                    double tolerance, int gate_sites) {
     const hipStream_t st = s->stream;
-    // occupancy grid (slot 0 of s->occ)
     const long long words = (s->V + 3) / 4;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(words)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(s->occ),
-                       words, 0u);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(s->bad), 1LL, 0u);
+    hipLaunchKernelGGL(occ_clear_kernel, dim3(grid_for(words)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(s->occ),
+                       words, s->bad);
     if (n > 0)
-        hipLaunchKernelGGL(occ_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, d_site, d_q, n, s->V, s->occ,
-                           s->bad);
+        hipLaunchKernelGGL(occ_scatter_f4_kernel, dim3((n + 255) / 256), dim3(256), 0, st, d_site, d_q, n, s->dims,
+                           s->occ, s->bad, s->chg);
     forward(s, s->w, s->occ, 1, d_map);
-    // exact pair sums at the gate sites
     const bool gauss = s->flags & LMG_GAUSS;
     const uint32_t c2m1 = lmg::cutoff_bits(s->p.cutoff, s->flags & LMG_NO_CUTOFF);
     const float gk = lmg::gauss_k(s->p.gaussian_sigma);
-    if (n > 0)
-        hipLaunchKernelGGL(lmg::to_float4_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->dims, d_site, d_q, n,
-                           s->chg, nullptr);
     const int S = lmg::split_count(gate_sites, n);
     const dim3 pgrid((gate_sites + lmg::kBlock - 1) / lmg::kBlock, S);
     if (gauss)
@@ -885,14 +985,10 @@ void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, 
     else
         hipLaunchKernelGGL((lmg::pair_kernel<false, false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
                            s->chg, n, c2m1, gk, s->gate_part, gate_sites);
-    const int per = lmg::kBlock / lmg::kRedLanes;
-    hipLaunchKernelGGL(lmg::pair_reduce_kernel, dim3((gate_sites + per - 1) / per), dim3(lmg::kBlock), 0, st,
-                       s->gate_part, gate_sites, S, gate_sites, s->p.prefactor, s->gate_exact);
-    const int nblk = std::min(kRedBlocks, (gate_sites + kRedThreads - 1) / kRedThreads);
-    hipLaunchKernelGGL(gate_part_kernel, dim3(nblk), dim3(kRedThreads), 0, st, d_map, s->gate_site, s->gate_exact,
-                       gate_sites, s->part);
-    hipLaunchKernelGGL(gate_final_kernel, dim3(1), dim3(kRedThreads), 0, st, s->part, nblk, tolerance, s->gate_err,
-                       s->gate_run);
+    const int per = 256 / kGateLanes;
+    hipLaunchKernelGGL(gate_reduce_kernel, dim3((gate_sites + per - 1) / per), dim3(256), 0, st, s->gate_part,
+                       gate_sites, S, s->p.prefactor, d_map, s->gate_site, s->part, s->gate_counter, tolerance,
+                       s->gate_err, s->gate_run);
     // exact map over the surrogate's when the gate failed (decided on the device)
     int tx = 1;
     while (tx < 64 && tx * lmg::kMapT < s->nx) tx <<= 1;
@@ -922,7 +1018,6 @@ lmg_status gate_buffers(lmg_surrogate* s, int n, int gate_sites, uint64_t gate_s
     if (gate_sites > s->gate_cap) {
         if ((st = regrow((void**)&s->gate_site, size_t(gate_sites) * 4)) ||
             (st = regrow((void**)&s->gate_tgt, size_t(gate_sites) * sizeof(float4))) ||
-            (st = regrow((void**)&s->gate_exact, size_t(gate_sites) * 8)) ||
             (st = regrow((void**)&s->gate_part, size_t(kGateMaxSplit) * gate_sites * sizeof(float2))))
             return st;
         s->gate_cap = gate_sites;
@@ -950,8 +1045,10 @@ lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const 
                                  double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
                                  int32_t* used_exact) {
     if (!s) return LMG_ERR_ARG;
-    if (n < 0 || !d_map || (n > 0 && (!d_site || !d_q)) || gate_sites < 1 || !(tolerance >= 0))
+    if (n < 0 || !d_map || (n > 0 && (!d_site || !d_q)) || !(tolerance >= 0))
         return sfail(s, LMG_ERR_ARG, "invalid map arguments");
+    if (gate_sites < 1 || gate_sites > kMaxGateSites)
+        return sfail(s, LMG_ERR_ARG, "gate_sites must be in [1, " + std::to_string(kMaxGateSites) + "]");
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
     if ((st = gate_buffers(s, n, gate_sites, gate_seed)) != LMG_OK) return st;

@@ -108,7 +108,7 @@ class Surrogate:
         return hist[:int(iterations)]
 
     def map_dev(self, site_ptr: int, q_ptr: int, n: int, map_ptr: int, tolerance: float = 1e-3,
-                gate_sites: int = 2048, gate_seed: int = 1, wait: bool = True):
+                gate_sites: int = 1024, gate_seed: int = 1, wait: bool = True):
         """Gated map; returns (sampled relative RMS error, exact fallback used),
         or None with wait=False (asynchronous; see :meth:`last_gate`)."""
         if not wait:
@@ -161,7 +161,7 @@ class Surrogate:
         g = d_g.download(np.float32, 2 * self.n_weights)
         return (g[0::2] + 1j * g[1::2]).reshape(self.spectrum_shape), loss
 
-    def map(self, sites, charges, tolerance: float = 1e-3, gate_sites: int = 2048,
+    def map(self, sites, charges, tolerance: float = 1e-3, gate_sites: int = 1024,
             gate_seed: int = 1) -> Tuple[np.ndarray, float, bool]:
         s = DeviceBuffer.from_array(np.asarray(sites, np.int32))
         q = DeviceBuffer.from_array(np.asarray(charges, np.int32))

@@ -13,7 +13,7 @@ C++ oracle restates as ``orc_potential_map``.  The surrogate network is
     map = window( irfftn( W * rfftn(q, s=P) , s=P ) )
 
 on a padded grid P (per axis >= N + c - 1, or >= 2N - 1 without a cutoff,
-rounded up to even; 1 for a flat axis), arrays in [z][y][x] order (site
+rounded up for alignment; 1 for a flat axis), arrays in [z][y][x] order (site
 s = x + X (y + Y z), cubicgrid.cpp:162-165).  ``irfftn`` here is numpy's: the
 imaginary parts of the kx = 0 and kx = Px/2 planes are discarded, exactly the
 half-spectrum convention of the device pipeline.
@@ -31,16 +31,24 @@ from typing import Tuple
 import numpy as np
 
 
-def padded_size(n: int, cutoff: int, no_cutoff: bool = False) -> int:
+def padded_size(n: int, cutoff: int, no_cutoff: bool = False, x_axis: bool = False) -> int:
+    """Padded length of one axis: at least N + c - 1 (2N - 1 without a cutoff),
+    which makes the circular convolution equal the aperiodic one; the x axis is
+    rounded up to 6 mod 8 (so Px / 2 + 1 is a multiple of 4) and y, z to a
+    multiple of 4, the alignment the device layouts use."""
     if n <= 1:
         return 1
     m = 2 * n - 1 if no_cutoff else min(2 * n - 1, n + cutoff - 1)
-    return m + (m & 1)
+    if x_axis:
+        return m + (6 - m) % 8
+    return m + (-m) % 4
 
 
 def padded_shape(dims, cutoff: int, no_cutoff: bool = False) -> Tuple[int, int, int]:
     """(Px, Py, Pz) for lattice dims (X, Y, Z)."""
-    return tuple(padded_size(int(n), int(cutoff), no_cutoff) for n in dims)
+    X, Y, Z = (int(n) for n in dims)
+    return (padded_size(X, cutoff, no_cutoff, True), padded_size(Y, cutoff, no_cutoff),
+            padded_size(Z, cutoff, no_cutoff))
 
 
 def spectrum_shape(dims, cutoff: int, no_cutoff: bool = False) -> Tuple[int, int, int]:

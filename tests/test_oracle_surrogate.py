@@ -47,7 +47,8 @@ def test_ideal_weights_reproduce_exact_map(oracle, dims, cutoff, sigma, none):
 
 
 def test_padding_rule_is_tight(oracle):
-    """P = N + c - 2 on x wraps in-cutoff pairs around; the rule's P does not."""
+    """A circular convolution of period P reproduces the map exactly for
+    P = N + c - 1 along x and wraps in-cutoff pairs around for P = N + c - 2."""
     dims, cutoff = (12, 6, 4), 5
     rng = np.random.default_rng(4)
     V = int(np.prod(dims))
@@ -55,18 +56,16 @@ def test_padding_rule_is_tight(oracle):
     occ = S.occupancy(dims, sites, q).astype(np.float64).reshape(4, 6, 12)
     ref = oracle.potential_map(dims, cutoff, sites, q, pref=PREF).reshape(4, 6, 12)
     px, py, pz = S.padded_shape(dims, cutoff)
-    assert px == 16            # 12 + 5 - 1 = 16
+    assert px >= 12 + cutoff - 1 and px % 8 == 6 and py % 4 == 0 and pz % 4 == 0
     K = S.circular_kernel(dims, cutoff, PREF)
-    for p_short in (px - 2,):  # 14 < N + c - 1
-        Kx = np.zeros((pz, py, p_short))
+    for p_x, exact in ((12 + cutoff - 1, True), (12 + cutoff - 2, False)):
+        Kx = np.zeros((pz, py, p_x))
         for dx in range(-(dims[0] - 1), dims[0]):
-            Kx[:, :, dx % p_short] += K[:, :, dx % px]
-        shp = (pz, py, p_short)
+            Kx[:, :, dx % p_x] += K[:, :, dx % px]
+        shp = (pz, py, p_x)
         out = np.fft.irfftn(np.fft.rfftn(Kx) * np.fft.rfftn(occ, s=shp, axes=(0, 1, 2)), s=shp, axes=(0, 1, 2))
-        assert np.max(np.abs(out[:4, :6, :12] - ref)) > 1e-3
-    shp = (pz, py, px)
-    out = np.fft.irfftn(np.fft.rfftn(K) * np.fft.rfftn(occ, s=shp, axes=(0, 1, 2)), s=shp, axes=(0, 1, 2))
-    assert np.max(np.abs(out[:4, :6, :12] - ref)) < 1e-12
+        err = np.max(np.abs(out[:4, :6, :12] - ref))
+        assert (err < 1e-12) if exact else (err > 1e-3)
 
 
 def test_gradient_matches_finite_differences():
@@ -107,7 +106,7 @@ def test_truncated_modes_cannot_meet_the_gate():
     err = np.linalg.norm(trunc - exact) / np.linalg.norm(exact)
     assert err > 0.1
     full_params = 2 * W.size
-    assert 1.5e6 < full_params < 2.0e6     # 64 x 160 x 81 complex weights
+    assert 1.5e6 < full_params < 2.0e6     # 64 x 160 x 80 complex weights
 
 
 def test_occupancy_sums_coincident_charges():
