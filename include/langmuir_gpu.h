@@ -168,6 +168,15 @@ lmg_status lmg_sim_store(lmg_ctx* ctx, lmg_sim_state* state);
  * the context is in synchronous mode (lmg_sim_set_sync). */
 lmg_status lmg_sim_step(lmg_ctx* ctx, int32_t n_steps);
 lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
+/* dE path for the following lmg_sim_load: 0 = automatic (culled when a group
+ * of targets plus the cutoff covers well under the lattice and the pair count
+ * is large), 1 = all pairs, 2 = culled (8^3-site brick lists).  Both give the
+ * same pair sums to FP32 rounding.  No counterpart in the reference, whose one
+ * path is the all-pairs loop of kernel.cl / ChargeAgent::coulombCPU
+ * (chargeagent.cpp:215-271).  lmg_sim_get_de_path reports the loaded path
+ * (1 or 2). */
+lmg_status lmg_sim_set_de_path(lmg_ctx* ctx, int32_t mode);
+lmg_status lmg_sim_get_de_path(const lmg_ctx* ctx, int32_t* mode);
 /* lmg_sim_step with HIP events around every kernel on the context stream;
  * returns once the steps are done.  kernel_ms[0] = summed dE-kernel time,
  * kernel_ms[1] = summed commit-kernel time, kernel_ms[2] = first-to-last

@@ -102,6 +102,8 @@ def lib():
     L.lmg_sim_store.argtypes = [VP, C.POINTER(LmgSimState)]
     L.lmg_sim_step.argtypes = [VP, I32]
     L.lmg_sim_set_sync.argtypes = [VP, I32]
+    L.lmg_sim_set_de_path.argtypes = [VP, I32]
+    L.lmg_sim_get_de_path.argtypes = [VP, C.POINTER(I32)]
     L.lmg_sim_delta_e.argtypes = [VP, VP, VP, VP, VP]
     L.lmg_sim_device_error.argtypes = [VP, PI32]
     L.lmg_sim_step_timed.argtypes = [VP, I32, PF64]
@@ -140,7 +142,8 @@ def lib():
 EXPORTED = (
     "lmg_create", "lmg_destroy", "lmg_last_error", "lmg_device_count", "lmg_sync", "lmg_stream",
     "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_potential_map", "lmg_potential_map_dev",
-    "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_delta_e",
+    "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_set_de_path",
+    "lmg_sim_get_de_path", "lmg_sim_delta_e",
     "lmg_sim_device_error", "lmg_sim_step_timed",
     "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
     "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_sync", "lmg_batch_store",

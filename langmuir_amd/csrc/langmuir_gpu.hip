@@ -58,6 +58,7 @@ struct lmg_ctx {
     // simulation
     bool sim_loaded = false;
     bool sync_mode = false;
+    int de_mode = 0;   // lmg_sim_set_de_path
     lmg_sim_config cfg{};
     SimDev sim{};
     SimDev* d_sim = nullptr;   // device copy the kernels read
@@ -194,7 +195,14 @@ unsigned long long next_pow2(unsigned long long v) {
 lmg_status sim_launch_de(lmg_ctx* c) {
     const SimDev& D = c->sim;
     const SimDev* P = c->d_sim;
-    if (!D.coulomb)
+    if (D.cull) {
+        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk), dim3(kSimBlock), 0, c->stream, P);
+        const dim3 grid((D.ngrp + 3) / 4 + 1, D.S);
+        if (D.gauss)
+            hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P);
+        else
+            hipLaunchKernelGGL((cull_de_kernel<false>), grid, dim3(kSimBlock), 0, c->stream, P);
+    } else if (!D.coulomb)
         hipLaunchKernelGGL((de_kernel<false, false>), dim3(D.nblk + 1, 1), dim3(kSimBlock), 0, c->stream, P);
     else if (D.gauss)
         hipLaunchKernelGGL((de_kernel<true, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
@@ -207,9 +215,15 @@ lmg_status sim_launch_de(lmg_ctx* c) {
 // has fewer than two neighbours)
 lmg_status sim_launch_propose(lmg_ctx* c) {
     const SimDev& D = c->sim;
+    // culled dE: propose_kernel also marks the carriers' bricks (cull_mark_kernel
+    // after serial proposals) and a kOneBlock propose_fix_kernel (cull_scan_kernel)
+    // turns the counts into offsets
     if (D.uniform) {
         hipLaunchKernelGGL(propose_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
-        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(64), 0, c->stream, c->d_sim);
+        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(D.cull ? kOneBlock : 64), 0, c->stream, c->d_sim);
+    } else if (D.cull) {
+        hipLaunchKernelGGL(cull_mark_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
+        hipLaunchKernelGGL(cull_scan_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim);
     }
     return launch_check(c, "step kernels");
 }
@@ -470,7 +484,8 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     std::memcpy(D.rate, cfg->rate, sizeof(D.rate));
     std::memcpy(D.coupling, cfg->coupling, sizeof(D.coupling));
     const int cap = D.cap_e + D.cap_h;
-    D.S = split_count(cap, cap + D.n_d_eff);
+    D.S_pairs = split_count(cap, cap + D.n_d_eff);
+    D.S = D.S_pairs;
     D.ld = cap;
     D.lookahead = 4ull * cap + 8192ull;
     const unsigned long long ring = next_pow2(2ull * (cap + D.lookahead + 4096ull));
@@ -478,6 +493,23 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     D.nblk = (cap + kSimBlock - 1) / kSimBlock;
     // drains on both x faces give every site at least two neighbours
     D.uniform = (D.e_drain[0] >= 0 && D.e_drain[1] >= 0 && D.h_drain[0] >= 0 && D.h_drain[1] >= 0) ? 1 : 0;
+    // culled dE when the cutoff box around a group of targets covers well under
+    // the whole lattice (estimated from a 16^3 group plus the cutoff)
+    D.cull = 0;
+    D.nbx = (d.X + kBrick - 1) / kBrick;
+    D.nby = (d.Y + kBrick - 1) / kBrick;
+    D.nbz = (d.Z + kBrick - 1) / kBrick;
+    D.nbrick = D.nbx * D.nby * D.nbz;
+    if (D.coulomb && D.nbrick <= kMaxCullBricks) {
+        auto frac = [&](int n) { return std::min(1.0, double(16 + 2 * c->p.cutoff) / n); };
+        const bool worth = frac(d.X) * frac(d.Y) * frac(d.Z) < 0.6 && double(cap) * (cap + D.n_d_eff) >= 8e6;
+        D.cull = c->de_mode == 2 || (c->de_mode == 0 && worth);
+    }
+    if (D.cull) {
+        D.cut = c->p.cutoff;
+        D.ngrp = (cap + 63) / 64;
+        D.S = std::max(4, std::min(kMaxCullS, (8192 + D.ngrp - 1) / D.ngrp));
+    }
 
     for (int b = 0; b < 2; ++b) {
         if ((st = sim_alloc(c, D.e_site[b], D.cap_e)) || (st = sim_alloc(c, D.e_life[b], D.cap_e)) ||
@@ -491,11 +523,65 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         (st = sim_alloc(c, D.h_at, d.V)) || (st = sim_alloc(c, D.code, cap)) ||
         (st = sim_alloc(c, D.removed, cap)) || (st = sim_alloc(c, D.flag, cap)) ||
         (st = sim_alloc(c, D.tgt1, cap)) || (st = sim_alloc(c, D.tgt2, cap)) ||
-        (st = sim_alloc(c, D.chg, cap + D.n_d_eff)) || (st = sim_alloc(c, D.part, size_t(D.S) * D.ld)) ||
+        (st = sim_alloc(c, D.chg, cap + D.n_d_eff)) ||
+        (st = sim_alloc(c, D.part, size_t(std::max(D.S, D.S_pairs)) * D.ld)) ||
         (st = sim_alloc(c, D.blk_draw, D.nblk)) || (st = sim_alloc(c, D.blk_keep, 2 * D.nblk)) ||
         (st = sim_alloc(c, D.de, cap)) || (st = sim_alloc(c, D.ring, ring)) || (st = sim_alloc(c, c->ctl, 1)) ||
         (st = sim_alloc(c, c->d_sim, 1)))
         return st;
+    if (D.cull) {
+        const int nb = D.nbrick;
+        int *bperm = nullptr, *dstart = nullptr;
+        float4* dsrc = nullptr;
+        if ((st = sim_alloc(c, D.bmask, size_t(nb) * 32)) || (st = sim_alloc(c, D.scnt, nb)) ||
+            (st = sim_alloc(c, D.tcnt, nb)) || (st = sim_alloc(c, D.tpos, cap)) ||
+            (st = sim_alloc(c, D.sstart, nb + 1)) || (st = sim_alloc(c, D.tstart, nb)) ||
+            (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
+            (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
+            (st = sim_alloc(c, D.torder, cap)))
+            return st;
+        HIPCHK(c, hipMemset(D.bmask, 0, size_t(nb) * 32 * 4));
+        HIPCHK(c, hipMemset(D.scnt, 0, size_t(nb) * 4));
+        HIPCHK(c, hipMemset(D.tcnt, 0, size_t(nb) * 4));
+        // bricks in Morton order of (bx, by, bz)
+        auto spread = [](uint64_t v) {
+            uint64_t r = 0;
+            for (int k = 0; k < 21; ++k) r |= ((v >> k) & 1ull) << (3 * k);
+            return r;
+        };
+        std::vector<std::pair<uint64_t, int>> key(nb);
+        for (int b = 0; b < nb; ++b) {
+            const int bx = b % D.nbx, by = (b / D.nbx) % D.nby, bz = b / (D.nbx * D.nby);
+            key[b] = {spread(bx) | (spread(by) << 1) | (spread(bz) << 2), b};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int> perm(nb);
+        for (int r = 0; r < nb; ++r) perm[r] = key[r].second;
+        HIPCHK(c, hipMemcpy(bperm, perm.data(), size_t(nb) * 4, hipMemcpyHostToDevice));
+        // charged defects by brick, then site, then list order
+        std::vector<std::pair<std::pair<int, int>, int>> dk(D.n_d_eff);
+        for (int k = 0; k < D.n_d_eff; ++k) {
+            int x, y, z;
+            decode(d, s->defect_site[k], x, y, z);
+            const int b = x / kBrick + D.nbx * (y / kBrick + D.nby * (z / kBrick));
+            dk[k] = {{b, (x & 7) + 8 * (y & 7) + 64 * (z & 7)}, k};
+        }
+        std::sort(dk.begin(), dk.end());
+        std::vector<int> ds(size_t(nb) + 1, 0);
+        std::vector<float4> dv(D.n_d_eff);
+        for (int k = 0; k < D.n_d_eff; ++k) {
+            ds[dk[k].first.first + 1]++;
+            int x, y, z;
+            decode(d, s->defect_site[dk[k].second], x, y, z);
+            dv[k] = make_float4(float(x), float(y), float(z), float(D.dq));
+        }
+        for (int b = 0; b < nb; ++b) ds[b + 1] += ds[b];
+        HIPCHK(c, hipMemcpy(dstart, ds.data(), ds.size() * 4, hipMemcpyHostToDevice));
+        if (D.n_d_eff > 0) HIPCHK(c, hipMemcpy(dsrc, dv.data(), dv.size() * sizeof(float4), hipMemcpyHostToDevice));
+        D.bperm = bperm;
+        D.dstart = dstart;
+        D.dsrc = dsrc;
+    }
     D.ctl = c->ctl;
     HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
 
@@ -549,6 +635,20 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     int32_t code = 0;
     if ((st = lmg_sim_device_error(c, &code)) != LMG_OK) return st;
     if (code) return fail(c, LMG_ERR_DEVICE, "device error " + std::to_string(code) + " while preparing");
+    return LMG_OK;
+}
+
+lmg_status lmg_sim_set_de_path(lmg_ctx* c, int32_t mode) {
+    if (!c) return LMG_ERR_ARG;
+    if (mode < 0 || mode > 2) return fail(c, LMG_ERR_ARG, "dE path must be 0 (automatic), 1 (all pairs) or 2 (culled)");
+    c->de_mode = mode;
+    return LMG_OK;
+}
+
+lmg_status lmg_sim_get_de_path(const lmg_ctx* c, int32_t* mode) {
+    if (!c || !mode) return LMG_ERR_ARG;
+    if (!c->sim_loaded) return LMG_ERR_STATE;
+    *mode = c->sim.cull ? 2 : 1;
     return LMG_OK;
 }
 
@@ -796,7 +896,8 @@ lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_c
     std::vector<SimDev> host(n);
     for (int r = 0; r < n; ++r) {
         SimDev& D = b->reps[r]->sim;
-        D.S = std::max(1, std::min(D.S, S_common));
+        D.cull = 0;   // the batch runs the all-pairs dE kernel over all replicas in one launch
+        D.S = std::max(1, std::min(D.S_pairs, S_common));
         host[r] = D;
         b->max_nblk = std::max(b->max_nblk, D.nblk);
         b->max_S = std::max(b->max_S, D.S);

@@ -58,6 +58,22 @@ struct SimDev {
     double rate[10];
     double coupling[27];
     int S, ld;
+    int S_pairs;                  // charge chunks of the all-pairs dE kernel
+    // culled dE (cull_*_kernel): 8^3-site bricks, targets in groups of 64
+    int cull;
+    int cut, ngrp;
+    int nbx, nby, nbz, nbrick;
+    uint32_t* bmask;              // [brick][electrons, holes][16] site bits (cleared by decide_kernel)
+    int* scnt;                    // carriers per brick (cull_mask_kernel; cleared by decide_kernel)
+    int* tcnt;                    // targets per brick (same)
+    int2* tpos;                   // per target: its brick, its slot in the brick
+    int* sstart;                  // [nbrick + 1] charge list offsets, padded to multiples of 4
+    int* tstart;                  // [nbrick] first target slot of each brick (bricks in Morton order)
+    const int* bperm;             // bricks in Morton order (static)
+    const int* dstart;            // [nbrick + 1] static charged-defect list offsets
+    const float4* dsrc;           // static charged defects, by brick then site
+    float4* src;                  // charge lists of the bricks
+    int* torder;                  // targets grouped by brick
     unsigned long long lookahead;
     int* e_site[2];
     int* e_life[2];
@@ -271,6 +287,381 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 }
 
 // ===========================================================================
+// Culled dE (cutoff well inside the lattice): the same pair sums as de_kernel
+// over only the charges near each target.
+//
+// The lattice is cut into bricks of 8 x 8 x 8 sites (partial at the far faces).
+// Every step the charges are listed brick by brick in a fixed order -- in a
+// brick: electrons, holes (each by site, from occupancy bitmasks), then the
+// static charged defects -- and each brick's list is padded with zero charges
+// to a multiple of 4.  Targets are grouped 64 to a wave in Morton order of
+// their bricks, so a wave's current and proposed sites fill a small box.
+//
+//   cull_mark (propose_kernel, or cull_mark_kernel after serial proposals):
+//     each carrier's bitmask bit, per-brick charge and target counts, and the
+//     carrier's slot among its brick's targets.
+//   propose_fix_kernel (or cull_scan_kernel): the counts become list offsets.
+//   cull_emit_kernel: one wave per brick writes its charge list; the other workgroups set the
+//     decision-draw flags (as de_kernel's chunk-0 workgroups) and put the
+//     targets into their brick's slots.
+//   cull_de_kernel: wave w, chunk s takes the bricks b with b mod S = s inside
+//     the box widened by the cutoff whose distance to the box is below the
+//     cutoff, and sums brick by brick: an FP32 sum over the brick's list, added with Kahan
+//     compensation when nonzero.  A skipped brick holds only charges at
+//     r1, r2 >= c for every target of the wave, whose terms are exactly 0, so
+//     each target's chunk sums are the same bits whatever its wave: the result
+//     is deterministic although the grouping (atomic scatter) is not.
+//   decide_kernel (committing pass) clears the bitmask words and brick
+//     counters each carrier set.
+// ===========================================================================
+constexpr int kBrick = 8;          // brick edge (sites)
+constexpr int kMaxCullBricks = 1 << 16;
+
+__device__ __forceinline__ int cull_brick(const SimDev& D, int x, int y, int z) {
+    return (x >> 3) + D.nbx * ((y >> 3) + D.nby * (z >> 3));
+}
+// bitmask word of a carrier (species 0 electrons, 1 holes) and its bit
+__device__ __forceinline__ int cull_word(const SimDev& D, int x, int y, int z, int sp, uint32_t& bit) {
+    const int j = (x & 7) + 8 * (y & 7) + 64 * (z & 7);
+    bit = 1u << (j & 31);
+    return (cull_brick(D, x, y, z) * 2 + sp) * 16 + (j >> 5);
+}
+
+__device__ __forceinline__ int load_agent(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// carrier i (current site) into its brick: bitmask bit, charge count, target slot
+__device__ __forceinline__ void cull_mark(const SimDev& D, int i, bool isE, int site) {
+    int x, y, z;
+    decode(D.d, site, x, y, z);
+    uint32_t bit;
+    atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
+    const int b = cull_brick(D, x, y, z);
+    atomicAdd(&D.scnt[b], 1);
+    D.tpos[i] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
+}
+
+// grid over carriers: cull_mark for every carrier (after inject_kernel made
+// the proposals serially; propose_kernel marks its carriers itself)
+__global__ __launch_bounds__(kSimBlock) void cull_mark_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
+    const int ne = D.ctl->n_e, nt = ne + D.ctl->n_h, cur = D.ctl->cur;
+    const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    if (i >= nt) return;
+    const bool isE = i < ne;
+    cull_mark(D, i, isE, isE ? D.e_site[cur][i] : D.h_site[cur][i - ne]);
+}
+
+// Offsets of the charge lists (linear brick order, counts padded to multiples
+// of 4; sst[0 .. nb]) and of the targets (bricks in Morton order; tst[brick])
+// by one workgroup of kThreads; thread t scans a contiguous range of bricks,
+// its first kPer counts loaded together.
+template <int kThreads>
+__device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
+    constexpr int kPer = 8;
+    const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
+    const int b0 = min(nb, int(threadIdx.x) * per), b1 = min(nb, b0 + per);
+    auto scount = [&](int b) { return (D.scnt[b] + D.dstart[b + 1] - D.dstart[b] + 3) & ~3; };
+    // unconditional loads (clamped index) so that all are in flight together
+    int bp[kPer], sc[kPer], tc[kPer];
+    int srun = 0, trun = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) bp[u] = D.bperm[min(b0 + u, nb - 1)];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int b = min(b0 + u, nb - 1);
+        sc[u] = scount(b);
+        tc[u] = D.tcnt[bp[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        if (b0 + u >= b1) sc[u] = tc[u] = 0;
+        srun += sc[u];
+        trun += tc[u];
+    }
+    for (int b = b0 + kPer; b < b1; ++b) {
+        srun += scount(b);
+        trun += D.tcnt[D.bperm[b]];
+    }
+    int soff, toff;
+    int so = wg_scan<kThreads>(srun, wsum, soff);
+    int to = wg_scan<kThreads>(trun, wsum, toff);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int b = b0 + u;
+        if (b < b1) {
+            sst[b] = so;
+            tst[bp[u]] = to;
+            so += sc[u];
+            to += tc[u];
+        }
+    }
+    for (int b = b0 + kPer; b < b1; ++b) {
+        const int q = D.bperm[b];
+        sst[b] = so;
+        tst[q] = to;
+        so += scount(b);
+        to += D.tcnt[q];
+    }
+    if (threadIdx.x == 0) sst[nb] = soff;
+}
+
+// the offsets after cull_mark_kernel (propose_fix_kernel computes them after
+// propose_kernel)
+__global__ __launch_bounds__(kOneBlock) void cull_scan_kernel(const SimDev* __restrict__ Dp) {
+    __shared__ int wsum[kOneBlock / 64];
+    cull_offsets<kOneBlock>(*Dp, Dp->sstart, Dp->tstart, wsum);
+}
+
+// grid = ceil(nbrick / 4) + nblk workgroups
+__global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
+    const int* sst = D.sstart;
+    const int* tst = D.tstart;
+    const int nbw = (D.nbrick + 3) / 4;
+    if (int(blockIdx.x) >= nbw) {   // targets: decision-draw flags, then into their brick's slots
+        const int ne = D.ctl->n_e, nt = D.ctl->counts[0];
+        const int i = (blockIdx.x - nbw) * kSimBlock + threadIdx.x;
+        int draw = 0;
+        if (i < nt) {
+            // decision draws: own-grid type Empty or Drain at the proposed site (chargeagent.cpp:95-157)
+            const bool isE = i < ne;
+            const int fs = isE ? D.e_fsite[i] : D.h_fsite[i - ne];
+            const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
+            draw = (tf == kEmpty || tf == kDrain);
+            D.flag[i] = draw;
+            const int2 tp = D.tpos[i];
+            D.torder[tst[tp.x] + tp.y] = i;
+        }
+        const int cnt = __syncthreads_count(draw);
+        if (threadIdx.x == 0) D.blk_draw[blockIdx.x - nbw] = cnt;
+        return;
+    }
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= D.nbrick) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t v = lane < 32 ? D.bmask[b * 32 + lane] : 0u;
+    const int cnt = __popc(v);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const int n = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += n;
+    }
+    const int neh = __shfl(incl, 31, 64);
+    if (lane == 0 && neh != D.scnt[b]) set_error(D, kErrSite);   // two of one species on a site
+    const int base = sst[b], end = sst[b + 1];
+    const int bx = b % D.nbx, byz = b / D.nbx, by = byz % D.nby, bz = byz / D.nby;
+    const float q = lane < 16 ? -1.f : 1.f;
+    int k = base + incl - cnt;
+    for (uint32_t m = v; m && k < end; m &= m - 1u, ++k) {
+        const int j = 32 * (lane & 15) + __builtin_ctz(m);
+        D.src[k] = make_float4(float(kBrick * bx + (j & 7)), float(kBrick * by + ((j >> 3) & 7)),
+                               float(kBrick * bz + (j >> 6)), q);
+    }
+    const int d0 = D.dstart[b], nd = D.dstart[b + 1] - d0;
+    for (int u = lane; u < nd && base + neh + u < end; u += 64) D.src[base + neh + u] = D.dsrc[d0 + u];
+    // zero charges far away pad the list (their terms are exactly 0)
+    for (int u = base + neh + nd + lane; u < end; u += 64) D.src[u] = make_float4(-1e4f, -1e4f, -1e4f, 0.f);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// t / d and t % d for 0 <= t < 2^24 with inv = 1 / d (float estimate, corrected)
+__device__ __forceinline__ int fdiv(int t, int d, float inv, int& r) {
+    int q = int(float(t) * inv);
+    r = t - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+    return q;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x - v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+constexpr int kList = 512;   // LDS charge slots per wave
+constexpr int kMaxCullS = 32;   // chunks
+constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
+
+// grid = (ceil(ngrp / 4) + 1, S): 4 target groups (waves) per workgroup;
+// workgroup (ceil(ngrp / 4), 0) refills the random stream.
+//
+// Wave (group g, chunk s):
+//   1. bricks: the bricks of the box widened by the cutoff are tested 64 at a
+//      time (lane per brick, in increasing brick order): kept if b mod S = s
+//      and the brick's distance to the box is below the cutoff; kept bricks
+//      are compacted into a list of up to 64.
+//   2. gather: lane j copies kept brick j's charges into the wave's LDS list
+//      (its own 64-byte rows, all lanes at once), at the exclusive scan of the
+//      counts; a brick that does not fit waits for the next round (a long one
+//      is split at 64-slot unit boundaries).
+//   3. sum: every lane runs over the list (broadcast reads): an FP32 sum per
+//      unit, added with Kahan compensation when nonzero.
+// The units and their order are the same for a target in any group, and the
+// units a group skips hold only terms that are exactly 0 for its targets.
+template <bool kGauss>
+__global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
+    const SimDev& D = *Dp;
+    __shared__ float4 lists[4][kList];
+    __shared__ int kbs[4][64];
+    __shared__ int uends[4][kList / 4];
+    const int ngb = (D.ngrp + 3) / 4;
+    if (int(blockIdx.x) >= ngb || int(blockIdx.y) >= D.S) {
+        if (int(blockIdx.x) == ngb && blockIdx.y == 0)
+            mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
+        return;
+    }
+    const int nt = D.ctl->counts[0];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = blockIdx.x * 4 + wv;
+    if (g * 64 >= nt) return;   // whole wave; nothing below synchronises the workgroup
+    int* kb = kbs[wv];
+    const int r = g * 64 + lane;
+    const bool valid = r < nt;
+    const int i = valid ? D.torder[r] : 0;
+    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f), p2 = p1;
+    if (valid) {
+        p1 = D.tgt1[i];
+        p2 = D.tgt2[i];
+    }
+    constexpr float kBig = 1e9f;
+    // the box is wave-uniform: keep it (and everything derived) in scalar registers
+    auto uni = [](float v) { return __builtin_amdgcn_readfirstlane(int(v)); };
+    const int x0 = uni(wave_min(valid ? fminf(p1.x, p2.x) : kBig));
+    const int y0 = uni(wave_min(valid ? fminf(p1.y, p2.y) : kBig));
+    const int z0 = uni(wave_min(valid ? fminf(p1.z, p2.z) : kBig));
+    const int x1 = uni(wave_max(valid ? fmaxf(p1.x, p2.x) : -kBig));
+    const int y1 = uni(wave_max(valid ? fmaxf(p1.y, p2.y) : -kBig));
+    const int z1 = uni(wave_max(valid ? fmaxf(p1.z, p2.z) : -kBig));
+    const int c = D.cut, c2 = c * c;
+    const int bx0 = max(0, x0 - c + 1) >> 3, bx1 = min(D.d.X - 1, x1 + c - 1) >> 3;
+    const int by0 = max(0, y0 - c + 1) >> 3, by1 = min(D.d.Y - 1, y1 + c - 1) >> 3;
+    const int bz0 = max(0, z0 - c + 1) >> 3, bz1 = min(D.d.Z - 1, z1 + c - 1) >> 3;
+    const int nx = bx1 - bx0 + 1, ny = by1 - by0 + 1, ncand = nx * ny * (bz1 - bz0 + 1);
+    const float inx = 1.f / float(nx), iny = 1.f / float(ny);
+    const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
+    const float hh = hx * hx + hy * hy + hz * hz;
+    const uint32_t c2m1 = D.c2m1;
+    const float gk = D.gk;
+    const int S = D.S, s = blockIdx.y;
+    const float iS = 1.f / float(S);
+    float sum = 0.f, comp = 0.f;
+
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    int cb = 0;
+    for (;;) {
+        // 1. kept bricks, up to 64, in increasing brick order
+        int nk = 0;
+        while (cb < ncand) {
+            const int ci = cb + lane;
+            bool keep = false;
+            int b = 0;
+            if (ci < ncand) {
+                int rx, ry;
+                const int ryz = fdiv(ci, nx, inx, rx);
+                const int rz = fdiv(ryz, ny, iny, ry);
+                const int bx = bx0 + rx, by = by0 + ry, bz = bz0 + rz;
+                const int gx = max(0, max(kBrick * bx - x1, x0 - (kBrick * bx + kBrick - 1)));
+                const int gy = max(0, max(kBrick * by - y1, y0 - (kBrick * by + kBrick - 1)));
+                const int gz = max(0, max(kBrick * bz - z1, z0 - (kBrick * bz + kBrick - 1)));
+                b = bx + D.nbx * (by + D.nby * bz);
+                int m;
+                (void)fdiv(b, S, iS, m);
+                keep = m == s && gx * gx + gy * gy + gz * gz < c2;
+            }
+            const unsigned long long bal = __ballot(keep);
+            const int nkeep = __popcll(bal);
+            if (nk + nkeep > 64) break;   // this batch again after the drain
+            const int rank =
+                __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
+            if (keep) kb[nk + rank] = b;
+            nk += nkeep;
+            cb += 64;
+        }
+        if (nk == 0) break;
+        __builtin_amdgcn_wave_barrier();
+        int k0 = 0, n = 0;
+        if (lane < nk) {
+            const int b = kb[lane];
+            k0 = D.sstart[b];
+            n = D.sstart[b + 1] - k0;
+        }
+        // 2 + 3, in rounds of at most kList slots
+        int done = 0;
+        for (;;) {
+            const int rem = n - done;
+            if (wave_sum(rem) == 0) break;
+            const int excl = wave_excl_scan(rem, lane);
+            int take = min(rem, max(0, kList - excl));
+            if (take < rem) take &= ~(kUnit - 1);   // whole units unless the brick ends here
+            const int nu = (take + kUnit - 1) / kUnit;
+            const int uex = wave_excl_scan(nu, lane);
+            const v4f* from = reinterpret_cast<const v4f*>(D.src) + (k0 + done);
+            v4f* to = reinterpret_cast<v4f*>(&lists[wv][0]) + excl;
+            int u = 0;
+            for (; u + 8 <= take; u += 8) {
+                const v4f a0 = from[u], a1 = from[u + 1], a2 = from[u + 2], a3 = from[u + 3];
+                const v4f a4 = from[u + 4], a5 = from[u + 5], a6 = from[u + 6], a7 = from[u + 7];
+                to[u] = a0; to[u + 1] = a1; to[u + 2] = a2; to[u + 3] = a3;
+                to[u + 4] = a4; to[u + 5] = a5; to[u + 6] = a6; to[u + 7] = a7;
+            }
+            if (u < take) {
+                const v4f a0 = from[u], a1 = from[u + 1], a2 = from[u + 2], a3 = from[u + 3];
+                to[u] = a0; to[u + 1] = a1; to[u + 2] = a2; to[u + 3] = a3;
+            }
+            for (int j = 0; j < nu; ++j) uends[wv][uex + j] = excl + min(kUnit * (j + 1), take);
+            done += take;
+            const int nunits = __builtin_amdgcn_readfirstlane(wave_sum(nu));
+            __builtin_amdgcn_wave_barrier();
+            int t = 0;
+            for (int q = 0; q < nunits; ++q) {
+                const int e = __builtin_amdgcn_readfirstlane(uends[wv][q]);
+                float acc = 0.f;
+                for (; t < e; t += 4) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float4 qc = lists[wv][t + j];
+                        acc = fmaf(qc.w, de_term<kGauss>(p1, qc, hx, hy, hz, hh, c2m1, gk), acc);
+                    }
+                }
+                if (acc != 0.f) kahan(sum, comp, acc);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (cb >= ncand) break;
+    }
+    if (valid) D.part[s * D.ld + i] = make_float2(sum, -comp);
+}
+
+// ===========================================================================
 // decide_kernel: decideFuture (chargeagent.cpp:90-161) for every carrier.
 // The draw of carrier i is word rng_pos + (draws of carriers before i).
 // only_de = 1 assembles dE (chargeagent.cpp:215-296) and stops.
@@ -302,13 +693,26 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     if (valid) {
         site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
         fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+        if (D.cull && !only_de) {   // the step is done with this carrier's bitmask word and brick counters
+            int x, y, z;
+            decode(D.d, site, x, y, z);
+            uint32_t bit;
+            D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)] = 0u;
+            const int b = cull_brick(D, x, y, z);
+            D.scnt[b] = 0;
+            D.tcnt[b] = 0;
+        }
     }
     double de = 0.0;
     if (valid && D.coulomb) {
-        double sde = 0.0;   // the S chunk partials, in chunk order (deterministic)
-        for (int q = 0; q < D.S; ++q) {
-            const float2 p = D.part[q * D.ld + i];
-            sde += double(p.x) + double(p.y);
+        double sde = 0.0;   // the S chunk partials, in chunk order (deterministic), 16 loads at a time
+        for (int q0 = 0; q0 < D.S; q0 += 16) {
+            float2 p[16];   // unconditional loads (clamped index) so that all are in flight together
+#pragma unroll
+            for (int u = 0; u < 16; ++u) p[u] = D.part[min(q0 + u, D.S - 1) * D.ld + i];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (q0 + u < D.S) sde += double(p[u].x) + double(p[u].y);
         }
         double bf, bs;   // bindingPotential (chargeagent.cpp:463-481)
         if (isE) {
@@ -855,13 +1259,18 @@ __global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __rest
     if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); return; }
     if (!propose_one(D, D.n_d_eff, i, isE, li, site, rng_word(D, C.rng_step0 + (unsigned long long)i)))
         atomicMin(&D.ctl->first_bad, i);
+    if (D.cull) cull_mark(D, i, isE, site);
 }
 
 // After propose_kernel: redo serially from the first rejection (if any) and
 // set the engine position of the pending decisions.
-__global__ void propose_fix_kernel(const SimDev* __restrict__ Dp) {
+__global__ __launch_bounds__(kOneBlock) void propose_fix_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];   // replica
     SimCtl& C = *D.ctl;
+    if (D.cull) {   // one workgroup of kOneBlock: the culled dE's offsets first
+        __shared__ int wsum[kOneBlock / 64];
+        cull_offsets<kOneBlock>(D, D.sstart, D.tstart, wsum);
+    }
     if (threadIdx.x != 0 || !D.uniform) return;
     const int ne = C.n_e, N = ne + C.n_h;
     const unsigned long long base = C.rng_step0;

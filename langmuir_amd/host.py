@@ -562,10 +562,14 @@ class Simulation:
                  potential: Optional[np.ndarray] = None, rng_window: Optional[np.ndarray] = None,
                  rng_consumed: int = 0, e_lifetime=None, e_pathlength=None, h_lifetime=None,
                  h_pathlength=None, flux_attempts=None, flux_successes=None, current_step: int = 0,
-                 device_id: int = 0, context: Optional[Context] = None):
+                 device_id: int = 0, context: Optional[Context] = None, de_path: str = "auto"):
         params.check()
         self.params = params
         self.ctx = context or Context.from_parameters(params, device_id)
+        modes = {"auto": 0, "pairs": 1, "culled": 2}
+        if de_path not in modes:
+            raise ValueError(f"de_path must be one of {sorted(modes)}")
+        self._chk(_abi.lib().lmg_sim_set_de_path(self.ctx.handle, modes[de_path]))
         self.config = make_config(params)
         self.cap_e = max(self.config.max_electrons, 1)
         self.cap_h = max(self.config.max_holes, 1)
@@ -585,6 +589,13 @@ class Simulation:
 
     def _chk(self, rc):
         self.ctx._chk(rc)
+
+    @property
+    def de_path(self) -> str:
+        """The dE path in use: "pairs" (every charge) or "culled" (brick lists)."""
+        m = C.c_int32(0)
+        self._chk(_abi.lib().lmg_sim_get_de_path(self.ctx.handle, C.byref(m)))
+        return "culled" if m.value == 2 else "pairs"
 
     def set_synchronous(self, on: bool = True):
         self._chk(_abi.lib().lmg_sim_set_sync(self.ctx.handle, int(on)))

@@ -56,19 +56,27 @@ CONFIGS = {
 }
 
 
-def make(oracle, gpu, name, **over):
+def make(oracle, gpu, name, de_path="auto", **over):
     kw = dict(CONFIGS[name])
     kw.update(over)
     op, sp = params_pair(**kw)
     w = oracle.World(op)
-    sim = gpu_from_oracle(gpu, w, sp)
+    sim = gpu_from_oracle(gpu, w, sp, de_path=de_path)
     sim.set_synchronous(True)
+    if de_path != "auto":
+        assert sim.de_path == (de_path if kw.get("coulomb_carriers") else "pairs")
     return w, sim, sp
 
 
+# every configuration through both dE paths: all pairs, and the culled brick
+# lists (forced here on lattices the automatic choice would run all-pairs)
+PATHS = ("pairs", "culled")
+
+
+@pytest.mark.parametrize("de_path", PATHS)
 @pytest.mark.parametrize("name", sorted(CONFIGS))
-def test_proposals_and_delta_e(gpu, oracle, name):
-    w, sim, sp = make(oracle, gpu, name)
+def test_proposals_and_delta_e(gpu, oracle, name, de_path):
+    w, sim, sp = make(oracle, gpu, name, de_path)
     fe, fh, de, dh = sim.delta_e()
     w.propose()
     w.coulomb()
@@ -101,20 +109,22 @@ def run_parity(w, sim, steps, beta):
     return mismatched
 
 
+@pytest.mark.parametrize("de_path", PATHS)
 @pytest.mark.parametrize("name,steps", [("transistor2d", 200), ("solar3d", 200), ("gauss3d", 80),
                                         ("defects", 150), ("hop2", 120), ("source_coulomb", 150),
                                         ("balance", 150), ("coulomb_off", 200)])
-def test_step_parity(gpu, oracle, name, steps):
-    w, sim, sp = make(oracle, gpu, name)
+def test_step_parity(gpu, oracle, name, steps, de_path):
+    w, sim, sp = make(oracle, gpu, name, de_path)
     mism = run_parity(w, sim, steps, sp.inverse_kt)
     print(f"{name}: {steps} steps, {mism} near-threshold resyncs")
     assert mism <= max(2, steps // 50)
     assert sim.device_error() == 0
 
 
-def test_transistor_full_size_parity(gpu, oracle):
+@pytest.mark.parametrize("de_path", PATHS)
+def test_transistor_full_size_parity(gpu, oracle, de_path):
     """The 256x256x1 transistor (1310 holes, 1 V, cutoff 50) for 25 steps."""
-    w, sim, sp = make(oracle, gpu, "transistor2d", grid_x=256, grid_y=256, random_seed=2024)
+    w, sim, sp = make(oracle, gpu, "transistor2d", de_path, grid_x=256, grid_y=256, random_seed=2024)
     assert w.info()["n_h"] == 1310
     mism = run_parity(w, sim, 25, sp.inverse_kt)
     assert mism <= 2
@@ -126,6 +136,7 @@ def test_solar_full_size_delta_e(gpu, oracle):
     w, sim, sp = make(oracle, gpu, "solar3d", grid_x=128, grid_y=128, grid_z=32,
                       electron_percentage=0.01, hole_percentage=0.01, seed_charges=1.0,
                       recombination_rate=0.0, generation_rate=0.524288, random_seed=77)
+    assert sim.de_path == "culled"   # the automatic choice at this size
     fe, fh, de, dh = sim.delta_e()
     w.propose()
     w.coulomb(threads=8)
