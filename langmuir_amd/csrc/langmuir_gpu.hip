@@ -508,7 +508,14 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     if (D.cull) {
         D.cut = c->p.cutoff;
         D.ngrp = (cap + 63) / 64;
-        D.S = std::max(4, std::min(kMaxCullS, (8192 + D.ngrp - 1) / D.ngrp));
+        // as many chunks per target group as keep every workgroup of
+        // cull_de_kernel resident at once (one wave of workgroups, no tail)
+        int ncu = 0, per_cu = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &per_cu, reinterpret_cast<const void*>(&cull_de_kernel<false>), kSimBlock, 0));
+        const int ngb = (D.ngrp + 3) / 4;
+        D.S = std::max(4, std::min(kMaxCullS, (std::max(1, ncu * per_cu) - 1) / ngb));
     }
 
     for (int b = 0; b < 2; ++b) {

@@ -508,30 +508,32 @@ __device__ __forceinline__ int wave_sum(int v) {
 }
 
 constexpr int kList = 512;   // LDS charge slots per wave
-constexpr int kMaxCullS = 32;   // chunks
+constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
+constexpr float kFix = 1099511627776.f;        // 2^40: fixed-point scale of the unit sums
+constexpr double kFixInv = 1.0 / 1099511627776.0;
 
 // grid = (ceil(ngrp / 4) + 1, S): 4 target groups (waves) per workgroup;
 // workgroup (ceil(ngrp / 4), 0) refills the random stream.
 //
-// Wave (group g, chunk s):
-//   1. bricks: the bricks of the box widened by the cutoff are tested 64 at a
-//      time (lane per brick, in increasing brick order): kept if b mod S = s
-//      and the brick's distance to the box is below the cutoff; kept bricks
-//      are compacted into a list of up to 64.
-//   2. gather: lane j copies kept brick j's charges into the wave's LDS list
-//      (its own 64-byte rows, all lanes at once), at the exclusive scan of the
-//      counts; a brick that does not fit waits for the next round (a long one
-//      is split at 64-slot unit boundaries).
-//   3. sum: every lane runs over the list (broadcast reads): an FP32 sum per
-//      unit, added with Kahan compensation when nonzero.
-// The units and their order are the same for a target in any group, and the
-// units a group skips hold only terms that are exactly 0 for its targets.
+// Every wave of group g finds the same work: the bricks of the box widened by
+// the cutoff whose distance to the box is below the cutoff (tested 64 at a
+// time, lane per brick), their lists cut into units (64-slot pieces), T slots
+// in all.  Chunk s takes the units whose first slot p (in candidate order) has
+// floor(p S / T) = s, so every chunk gets T / S slots give or take a unit:
+//   gather: each lane copies its brick's units of the chunk into the wave's LDS
+//     list (its own 64-byte rows, all lanes at once) at the exclusive scan of
+//     the counts; the list is summed whenever the next bricks do not fit;
+//   sum: every lane runs over the list (broadcast reads): an FP32 sum per unit,
+//     taken to fixed point (x 2^40, truncated) and added in int64.
+// The units and their FP32 sums are the same for a target in any group, and
+// the units a group skips hold only terms that are exactly 0 for its targets;
+// integer addition is exact, so each target's total is the same whatever its
+// group, chunk count or order.
 template <bool kGauss>
 __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = *Dp;
     __shared__ float4 lists[4][kList];
-    __shared__ int kbs[4][64];
     __shared__ int uends[4][kList / 4];
     const int ngb = (D.ngrp + 3) / 4;
     if (int(blockIdx.x) >= ngb || int(blockIdx.y) >= D.S) {
@@ -543,7 +545,6 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = blockIdx.x * 4 + wv;
     if (g * 64 >= nt) return;   // whole wave; nothing below synchronises the workgroup
-    int* kb = kbs[wv];
     const int r = g * 64 + lane;
     const bool valid = r < nt;
     const int i = valid ? D.torder[r] : 0;
@@ -572,93 +573,107 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
     const int S = D.S, s = blockIdx.y;
-    const float iS = 1.f / float(S);
-    float sum = 0.f, comp = 0.f;
+    const int* __restrict__ sstart = D.sstart;
+    // candidate ci -> its list [k0, k0 + n) when kept, else n = 0
+    auto brick = [&](int ci, int& k0) -> int {
+        k0 = 0;
+        if (ci >= ncand) return 0;
+        int rx, ry;
+        const int ryz = fdiv(ci, nx, inx, rx);
+        const int rz = fdiv(ryz, ny, iny, ry);
+        const int bx = bx0 + rx, by = by0 + ry, bz = bz0 + rz;
+        const int gx = max(0, max(kBrick * bx - x1, x0 - (kBrick * bx + kBrick - 1)));
+        const int gy = max(0, max(kBrick * by - y1, y0 - (kBrick * by + kBrick - 1)));
+        const int gz = max(0, max(kBrick * bz - z1, z0 - (kBrick * bz + kBrick - 1)));
+        if (gx * gx + gy * gy + gz * gz >= c2) return 0;
+        const int b = bx + D.nbx * (by + D.nby * bz);
+        k0 = sstart[b];
+        return sstart[b + 1] - k0;
+    };
+    // T: every kept slot, 8 candidates per lane in flight
+    int tl = 0;
+    for (int cb = 0; cb < ncand; cb += 512) {
+        int n8[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            int k0;
+            n8[j] = brick(cb + 64 * j + lane, k0);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) tl += n8[j];
+    }
+    const int T = __builtin_amdgcn_readfirstlane(wave_sum(tl));
+    // this chunk's slot positions: [lo, hi)
+    const long long sT = (long long)s * T, s1T = (long long)(s + 1) * T;
+    const int lo = int((sT + S - 1) / S), hi = int((s1T + S - 1) / S);
 
     typedef float v4f __attribute__((ext_vector_type(4)));
-    int cb = 0;
-    for (;;) {
-        // 1. kept bricks, up to 64, in increasing brick order
-        int nk = 0;
-        while (cb < ncand) {
-            const int ci = cb + lane;
-            bool keep = false;
-            int b = 0;
-            if (ci < ncand) {
-                int rx, ry;
-                const int ryz = fdiv(ci, nx, inx, rx);
-                const int rz = fdiv(ryz, ny, iny, ry);
-                const int bx = bx0 + rx, by = by0 + ry, bz = bz0 + rz;
-                const int gx = max(0, max(kBrick * bx - x1, x0 - (kBrick * bx + kBrick - 1)));
-                const int gy = max(0, max(kBrick * by - y1, y0 - (kBrick * by + kBrick - 1)));
-                const int gz = max(0, max(kBrick * bz - z1, z0 - (kBrick * bz + kBrick - 1)));
-                b = bx + D.nbx * (by + D.nby * bz);
-                int m;
-                (void)fdiv(b, S, iS, m);
-                keep = m == s && gx * gx + gy * gy + gz * gz < c2;
-            }
-            const unsigned long long bal = __ballot(keep);
-            const int nkeep = __popcll(bal);
-            if (nk + nkeep > 64) break;   // this batch again after the drain
-            const int rank =
-                __builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u));
-            if (keep) kb[nk + rank] = b;
-            nk += nkeep;
-            cb += 64;
-        }
-        if (nk == 0) break;
+    long long acc64 = 0;
+    int fill = 0, nu = 0, pos = 0;   // list slots and units in use; slot position of the batch
+    auto flush = [&]() {
         __builtin_amdgcn_wave_barrier();
-        int k0 = 0, n = 0;
-        if (lane < nk) {
-            const int b = kb[lane];
-            k0 = D.sstart[b];
-            n = D.sstart[b + 1] - k0;
+        int t = 0;
+        for (int q = 0; q < nu; ++q) {
+            const int e = __builtin_amdgcn_readfirstlane(uends[wv][q]);
+            float acc = 0.f;
+            for (; t < e; t += 4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float4 qc = lists[wv][t + j];
+                    acc = fmaf(qc.w, de_term<kGauss>(p1, qc, hx, hy, hz, hh, c2m1, gk), acc);
+                }
+            }
+            acc64 += (long long)(acc * kFix);
         }
-        // 2 + 3, in rounds of at most kList slots
-        int done = 0;
+        __builtin_amdgcn_wave_barrier();
+        fill = 0;
+        nu = 0;
+    };
+    for (int cb = 0; cb < ncand && pos < hi; cb += 64) {
+        int k0;
+        const int n = brick(cb + lane, k0);
+        const int pex = pos + wave_excl_scan(n, lane);   // first slot position of this lane's brick
+        pos += __builtin_amdgcn_readfirstlane(wave_sum(n));
+        // this chunk's units of the brick: unit j starts at pex + 64 j
+        const int nun = (n + kUnit - 1) / kUnit;
+        int u0 = 0, u1 = 0;
+        if (n > 0) {
+            u0 = min(nun, max(0, lo - pex + kUnit - 1) / kUnit);
+            u1 = min(nun, max(0, hi - pex + kUnit - 1) / kUnit);
+        }
+        int from = kUnit * u0, todo = max(0, min(n, kUnit * u1) - from);   // slots of the brick to take
         for (;;) {
-            const int rem = n - done;
-            if (wave_sum(rem) == 0) break;
-            const int excl = wave_excl_scan(rem, lane);
-            int take = min(rem, max(0, kList - excl));
-            if (take < rem) take &= ~(kUnit - 1);   // whole units unless the brick ends here
-            const int nu = (take + kUnit - 1) / kUnit;
-            const int uex = wave_excl_scan(nu, lane);
-            const v4f* from = reinterpret_cast<const v4f*>(D.src) + (k0 + done);
-            v4f* to = reinterpret_cast<v4f*>(&lists[wv][0]) + excl;
+            const int excl = wave_excl_scan(todo, lane);
+            const int total = __builtin_amdgcn_readfirstlane(wave_sum(todo));
+            if (total == 0) break;
+            if (fill + total > kList && fill > 0) flush();
+            // whole units that fit (a lane's first unit always fits after a flush)
+            int take = min(todo, max(0, kList - fill - excl));
+            if (take < todo) take &= ~(kUnit - 1);
+            const int nut = (take + kUnit - 1) / kUnit;
+            const int uex = wave_excl_scan(nut, lane);
+            const v4f* src = reinterpret_cast<const v4f*>(D.src) + (k0 + from);
+            v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]) + (fill + excl);
             int u = 0;
             for (; u + 8 <= take; u += 8) {
-                const v4f a0 = from[u], a1 = from[u + 1], a2 = from[u + 2], a3 = from[u + 3];
-                const v4f a4 = from[u + 4], a5 = from[u + 5], a6 = from[u + 6], a7 = from[u + 7];
-                to[u] = a0; to[u + 1] = a1; to[u + 2] = a2; to[u + 3] = a3;
-                to[u + 4] = a4; to[u + 5] = a5; to[u + 6] = a6; to[u + 7] = a7;
+                const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
+                const v4f a4 = src[u + 4], a5 = src[u + 5], a6 = src[u + 6], a7 = src[u + 7];
+                dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
+                dst[u + 4] = a4; dst[u + 5] = a5; dst[u + 6] = a6; dst[u + 7] = a7;
             }
             if (u < take) {
-                const v4f a0 = from[u], a1 = from[u + 1], a2 = from[u + 2], a3 = from[u + 3];
-                to[u] = a0; to[u + 1] = a1; to[u + 2] = a2; to[u + 3] = a3;
+                const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
+                dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
             }
-            for (int j = 0; j < nu; ++j) uends[wv][uex + j] = excl + min(kUnit * (j + 1), take);
-            done += take;
-            const int nunits = __builtin_amdgcn_readfirstlane(wave_sum(nu));
-            __builtin_amdgcn_wave_barrier();
-            int t = 0;
-            for (int q = 0; q < nunits; ++q) {
-                const int e = __builtin_amdgcn_readfirstlane(uends[wv][q]);
-                float acc = 0.f;
-                for (; t < e; t += 4) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float4 qc = lists[wv][t + j];
-                        acc = fmaf(qc.w, de_term<kGauss>(p1, qc, hx, hy, hz, hh, c2m1, gk), acc);
-                    }
-                }
-                if (acc != 0.f) kahan(sum, comp, acc);
-            }
-            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < nut; ++j) uends[wv][nu + uex + j] = fill + excl + min(kUnit * (j + 1), take);
+            fill += __builtin_amdgcn_readfirstlane(wave_sum(take));
+            nu += __builtin_amdgcn_readfirstlane(wave_sum(nut));
+            from += take;
+            todo -= take;
         }
-        if (cb >= ncand) break;
     }
-    if (valid) D.part[s * D.ld + i] = make_float2(sum, -comp);
+    flush();
+    if (valid) reinterpret_cast<long long*>(D.part)[s * D.ld + i] = acc64;
 }
 
 // ===========================================================================
@@ -706,7 +721,20 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     double de = 0.0;
     if (valid && D.coulomb) {
         double sde = 0.0;   // the S chunk partials, in chunk order (deterministic), 16 loads at a time
-        for (int q0 = 0; q0 < D.S; q0 += 16) {
+        if (D.cull) {   // fixed-point partials of cull_de_kernel: exact integer sum
+            const long long* ip = reinterpret_cast<const long long*>(D.part);
+            long long tot = 0;
+            for (int q0 = 0; q0 < D.S; q0 += 16) {
+                long long v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = ip[min(q0 + u, D.S - 1) * D.ld + i];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (q0 + u < D.S) tot += v[u];
+            }
+            sde = double(tot) * kFixInv;
+        }
+        for (int q0 = 0; q0 < (D.cull ? 0 : D.S); q0 += 16) {
             float2 p[16];   // unconditional loads (clamped index) so that all are in flight together
 #pragma unroll
             for (int u = 0; u < 16; ++u) p[u] = D.part[min(q0 + u, D.S - 1) * D.ld + i];
