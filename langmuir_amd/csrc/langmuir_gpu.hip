@@ -196,8 +196,8 @@ lmg_status sim_launch_de(lmg_ctx* c) {
     const SimDev& D = c->sim;
     const SimDev* P = c->d_sim;
     if (D.cull) {
-        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk), dim3(kSimBlock), 0, c->stream, P);
-        const dim3 grid((D.ngrp + 3) / 4 + 1, D.S);
+        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream, P);
+        const dim3 grid((D.items_cap + 3) / 4 + 1);
         if (D.gauss)
             hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P);
         else
@@ -514,8 +514,9 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                       &per_cu, reinterpret_cast<const void*>(&cull_de_kernel<false>), kSimBlock, 0));
-        const int ngb = (D.ngrp + 3) / 4;
-        D.S = std::max(4, std::min(kMaxCullS, (std::max(1, ncu * per_cu) - 1) / ngb));
+        // work items: as many as keep every workgroup resident at once (one
+        // wave of workgroups, no tail), at least one per target group
+        D.items_cap = std::max(4 * (ncu * per_cu - 1), D.ngrp);
     }
 
     for (int b = 0; b < 2; ++b) {
@@ -545,11 +546,17 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
             (st = sim_alloc(c, D.sstart, nb + 1)) || (st = sim_alloc(c, D.tstart, nb)) ||
             (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
             (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
-            (st = sim_alloc(c, D.torder, cap)))
+            (st = sim_alloc(c, D.torder, cap)) || (st = sim_alloc(c, D.tgrp, D.ngrp)) ||
+            (st = sim_alloc(c, D.items, D.items_cap)) || (st = sim_alloc(c, D.nitems, 1)) ||
+            (st = sim_alloc(c, D.isum, cap)))
             return st;
         HIPCHK(c, hipMemset(D.bmask, 0, size_t(nb) * 32 * 4));
         HIPCHK(c, hipMemset(D.scnt, 0, size_t(nb) * 4));
         HIPCHK(c, hipMemset(D.tcnt, 0, size_t(nb) * 4));
+        {
+            const std::vector<int> ones(D.ngrp, 1);   // no culled pass yet: equal shares
+            HIPCHK(c, hipMemcpy(D.tgrp, ones.data(), ones.size() * 4, hipMemcpyHostToDevice));
+        }
         // bricks in Morton order of (bx, by, bz)
         auto spread = [](uint64_t v) {
             uint64_t r = 0;

@@ -74,6 +74,11 @@ struct SimDev {
     const float4* dsrc;           // static charged defects, by brick then site
     float4* src;                  // charge lists of the bricks
     int* torder;                  // targets grouped by brick
+    int* tgrp;                    // slots of each target group in the last culled pass (>= 1)
+    int4* items;                  // cull_de_kernel's work items: (group, chunk, chunks of the group)
+    int* nitems;
+    int items_cap;
+    long long* isum;              // per target: fixed-point pair sum (int64 atomics)
     unsigned long long lookahead;
     int* e_site[2];
     int* e_life[2];
@@ -414,17 +419,54 @@ __global__ __launch_bounds__(kOneBlock) void cull_scan_kernel(const SimDev* __re
     cull_offsets<kOneBlock>(*Dp, Dp->sstart, Dp->tstart, wsum);
 }
 
-// grid = ceil(nbrick / 4) + nblk workgroups
+// Work items of cull_de_kernel, one workgroup of kSimBlock: group g gets
+// S_g = max(1, round(K T_g / sum T)) chunks, T_g its slots in the previous
+// culled pass (the groups' regions move by about a site per step), K the
+// number of item slots left after one item per group; items are (g, chunk,
+// S_g) in group order.  Any S_g gives the same sums: it only balances the work.
+__device__ void cull_items(const SimDev& D, int* wsum) {
+    const int ng = (D.ctl->counts[0] + 63) / 64;
+    const int per = (ng + kSimBlock - 1) / kSimBlock;
+    const int g0 = min(ng, int(threadIdx.x) * per), g1 = min(ng, g0 + per);
+    long long wl = 0;
+    for (int g = g0; g < g1; ++g) wl += D.tgrp[g];
+    // sum T over the workgroup (wave sums, then the four waves)
+    for (int o = 32; o > 0; o >>= 1) wl += __shfl_xor(wl, o, 64);
+    __shared__ long long wtot[4];
+    if ((threadIdx.x & 63) == 0) wtot[threadIdx.x >> 6] = wl;
+    __syncthreads();
+    const long long W = max(1ll, wtot[0] + wtot[1] + wtot[2] + wtot[3]);
+    const long long K = max(0, D.items_cap - ng);
+    auto chunks = [&](int g) { return 1 + int((K * D.tgrp[g]) / W); };
+    int run = 0;
+    for (int g = g0; g < g1; ++g) run += chunks(g);
+    int total;
+    int k = wg_scan<kSimBlock>(run, wsum, total);
+    for (int g = g0; g < g1; ++g) {
+        const int sg = chunks(g);
+        for (int c = 0; c < sg; ++c) D.items[k + c] = make_int4(g, c, sg, 0);
+        k += sg;
+    }
+    if (threadIdx.x == 0) *D.nitems = total;
+}
+
+// grid = ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
 __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = *Dp;
     const int* sst = D.sstart;
     const int* tst = D.tstart;
     const int nbw = (D.nbrick + 3) / 4;
+    if (blockIdx.x == gridDim.x - 1) {   // cull_de_kernel's work items
+        __shared__ int wsum[4];
+        cull_items(D, wsum);
+        return;
+    }
     if (int(blockIdx.x) >= nbw) {   // targets: decision-draw flags, then into their brick's slots
         const int ne = D.ctl->n_e, nt = D.ctl->counts[0];
         const int i = (blockIdx.x - nbw) * kSimBlock + threadIdx.x;
         int draw = 0;
         if (i < nt) {
+            D.isum[i] = 0;
             // decision draws: own-grid type Empty or Drain at the proposed site (chargeagent.cpp:95-157)
             const bool isE = i < ne;
             const int fs = isE ? D.e_fsite[i] : D.h_fsite[i - ne];
@@ -507,25 +549,57 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
+// Wave-wide prefix sums and reductions with DPP moves (no LDS traffic):
+// row_shr:1,2,4,8 scan each row of 16 lanes, row_bcast:15 and row_bcast:31
+// carry the row totals into the rows above.  Lanes a move has no source for
+// (and rows its row mask leaves out) take the identity.
+template <class Op>
+__device__ __forceinline__ int dpp_scan(int x, int id, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));   // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));   // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));   // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));   // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));   // row_bcast:15 -> rows 1, 3
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));   // row_bcast:31 -> rows 2, 3
+    return x;
+}
+__device__ __forceinline__ int wave_incl_sum(int x) {
+    return dpp_scan(x, 0, [](int a, int b) { return a + b; });
+}
+// the total, wave-uniform (lane 63 of the inclusive scan)
+__device__ __forceinline__ int wave_total(int incl) { return __builtin_amdgcn_readlane(incl, 63); }
+__device__ __forceinline__ float wave_fmin(float v) {
+    const int r = dpp_scan(__float_as_int(v), __float_as_int(3.0e38f),
+                           [](int a, int b) { return __float_as_int(fminf(__int_as_float(a), __int_as_float(b))); });
+    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
+}
+__device__ __forceinline__ float wave_fmax(float v) {
+    const int r = dpp_scan(__float_as_int(v), __float_as_int(-3.0e38f),
+                           [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); });
+    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
+}
+
 constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
 constexpr float kFix = 1099511627776.f;        // 2^40: fixed-point scale of the unit sums
 constexpr double kFixInv = 1.0 / 1099511627776.0;
 
-// grid = (ceil(ngrp / 4) + 1, S): 4 target groups (waves) per workgroup;
-// workgroup (ceil(ngrp / 4), 0) refills the random stream.
+// grid = ceil(items_cap / 4) + 1 workgroups: wave w of workgroup x takes work
+// item 4x + w (group g, chunk s of S); the last workgroup refills the random
+// stream.
 //
 // Every wave of group g finds the same work: the bricks of the box widened by
 // the cutoff whose distance to the box is below the cutoff (tested 64 at a
 // time, lane per brick), their lists cut into units (64-slot pieces), T slots
 // in all.  Chunk s takes the units whose first slot p (in candidate order) has
 // floor(p S / T) = s, so every chunk gets T / S slots give or take a unit:
-//   gather: each lane copies its brick's units of the chunk into the wave's LDS
-//     list (its own 64-byte rows, all lanes at once) at the exclusive scan of
-//     the counts; the list is summed whenever the next bricks do not fit;
+//   gather: each lane copies its bricks' units of the chunk into the wave's
+//     LDS list (its own 64-byte rows, all lanes at once) at the exclusive scan
+//     of the counts; the list is summed whenever the next bricks do not fit;
 //   sum: every lane runs over the list (broadcast reads): an FP32 sum per unit,
-//     taken to fixed point (x 2^40, truncated) and added in int64.
+//     taken to fixed point (x 2^40, truncated) and added in int64; the chunk's
+//     total goes into the target's isum with a 64-bit integer atomic.
 // The units and their FP32 sums are the same for a target in any group, and
 // the units a group skips hold only terms that are exactly 0 for its targets;
 // integer addition is exact, so each target's total is the same whatever its
@@ -535,16 +609,16 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const SimDev& D = *Dp;
     __shared__ float4 lists[4][kList];
     __shared__ int uends[4][kList / 4];
-    const int ngb = (D.ngrp + 3) / 4;
-    if (int(blockIdx.x) >= ngb || int(blockIdx.y) >= D.S) {
-        if (int(blockIdx.x) == ngb && blockIdx.y == 0)
-            mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
+    if (blockIdx.x == gridDim.x - 1) {
+        mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     const int nt = D.ctl->counts[0];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int g = blockIdx.x * 4 + wv;
-    if (g * 64 >= nt) return;   // whole wave; nothing below synchronises the workgroup
+    const int item = blockIdx.x * 4 + wv;
+    if (item >= *D.nitems) return;   // whole wave; nothing below synchronises the workgroup
+    const int4 it = D.items[item];
+    const int g = it.x, s = it.y, S = it.z;
     const int r = g * 64 + lane;
     const bool valid = r < nt;
     const int i = valid ? D.torder[r] : 0;
@@ -554,14 +628,13 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         p2 = D.tgt2[i];
     }
     constexpr float kBig = 1e9f;
-    // the box is wave-uniform: keep it (and everything derived) in scalar registers
-    auto uni = [](float v) { return __builtin_amdgcn_readfirstlane(int(v)); };
-    const int x0 = uni(wave_min(valid ? fminf(p1.x, p2.x) : kBig));
-    const int y0 = uni(wave_min(valid ? fminf(p1.y, p2.y) : kBig));
-    const int z0 = uni(wave_min(valid ? fminf(p1.z, p2.z) : kBig));
-    const int x1 = uni(wave_max(valid ? fmaxf(p1.x, p2.x) : -kBig));
-    const int y1 = uni(wave_max(valid ? fmaxf(p1.y, p2.y) : -kBig));
-    const int z1 = uni(wave_max(valid ? fmaxf(p1.z, p2.z) : -kBig));
+    // the box is wave-uniform (scalar registers)
+    const int x0 = int(wave_fmin(valid ? fminf(p1.x, p2.x) : kBig));
+    const int y0 = int(wave_fmin(valid ? fminf(p1.y, p2.y) : kBig));
+    const int z0 = int(wave_fmin(valid ? fminf(p1.z, p2.z) : kBig));
+    const int x1 = int(wave_fmax(valid ? fmaxf(p1.x, p2.x) : -kBig));
+    const int y1 = int(wave_fmax(valid ? fmaxf(p1.y, p2.y) : -kBig));
+    const int z1 = int(wave_fmax(valid ? fmaxf(p1.z, p2.z) : -kBig));
     const int c = D.cut, c2 = c * c;
     const int bx0 = max(0, x0 - c + 1) >> 3, bx1 = min(D.d.X - 1, x1 + c - 1) >> 3;
     const int by0 = max(0, y0 - c + 1) >> 3, by1 = min(D.d.Y - 1, y1 + c - 1) >> 3;
@@ -572,7 +645,6 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const float hh = hx * hx + hy * hy + hz * hz;
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
-    const int S = D.S, s = blockIdx.y;
     const int* __restrict__ sstart = D.sstart;
     // candidate ci -> its list [k0, k0 + n) when kept, else n = 0
     auto brick = [&](int ci, int& k0) -> int {
@@ -590,26 +662,20 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         k0 = sstart[b];
         return sstart[b + 1] - k0;
     };
-    // T: every kept slot, 8 candidates per lane in flight
-    int tl = 0;
-    for (int cb = 0; cb < ncand; cb += 512) {
-        int n8[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            int k0;
-            n8[j] = brick(cb + 64 * j + lane, k0);
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) tl += n8[j];
-    }
-    const int T = __builtin_amdgcn_readfirstlane(wave_sum(tl));
-    // this chunk's slot positions: [lo, hi)
-    const long long sT = (long long)s * T, s1T = (long long)(s + 1) * T;
-    const int lo = int((sT + S - 1) / S), hi = int((s1T + S - 1) / S);
+    // this chunk's units of a brick whose first slot has position pex: slots
+    // [from, from + todo) of its list (unit j starts at pex + 64 j)
+    int lo = 0, hi = 0;
+    auto owned = [&](int n, int pex, int& from) -> int {
+        const int nun = (n + kUnit - 1) / kUnit;
+        const int u0 = min(nun, max(0, lo - pex + kUnit - 1) / kUnit);
+        const int u1 = min(nun, max(0, hi - pex + kUnit - 1) / kUnit);
+        from = kUnit * u0;
+        return n > 0 ? max(0, min(n, kUnit * u1) - from) : 0;
+    };
 
     typedef float v4f __attribute__((ext_vector_type(4)));
     long long acc64 = 0;
-    int fill = 0, nu = 0, pos = 0;   // list slots and units in use; slot position of the batch
+    int fill = 0, nu = 0;   // list slots and units in use
     auto flush = [&]() {
         __builtin_amdgcn_wave_barrier();
         int t = 0;
@@ -629,51 +695,106 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         fill = 0;
         nu = 0;
     };
-    for (int cb = 0; cb < ncand && pos < hi; cb += 64) {
-        int k0;
-        const int n = brick(cb + lane, k0);
-        const int pex = pos + wave_excl_scan(n, lane);   // first slot position of this lane's brick
-        pos += __builtin_amdgcn_readfirstlane(wave_sum(n));
-        // this chunk's units of the brick: unit j starts at pex + 64 j
-        const int nun = (n + kUnit - 1) / kUnit;
-        int u0 = 0, u1 = 0;
-        if (n > 0) {
-            u0 = min(nun, max(0, lo - pex + kUnit - 1) / kUnit);
-            u1 = min(nun, max(0, hi - pex + kUnit - 1) / kUnit);
+    // copy `take` slots (whole units, or up to the brick's end) from list slot
+    // k of the charge lists to the wave's list at slot o, with their unit ends
+    auto copy = [&](int k, int take, int o, int uo) {
+        const v4f* src = reinterpret_cast<const v4f*>(D.src) + k;
+        v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]) + o;
+        int u = 0;
+        for (; u + 8 <= take; u += 8) {
+            const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
+            const v4f a4 = src[u + 4], a5 = src[u + 5], a6 = src[u + 6], a7 = src[u + 7];
+            dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
+            dst[u + 4] = a4; dst[u + 5] = a5; dst[u + 6] = a6; dst[u + 7] = a7;
         }
-        int from = kUnit * u0, todo = max(0, min(n, kUnit * u1) - from);   // slots of the brick to take
+        if (u < take) {
+            const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
+            dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
+        }
+        for (int j = 0; j * kUnit < take; ++j) uends[wv][uo + j] = o + min(kUnit * (j + 1), take);
+    };
+    // one brick segment per lane, [k, k + todo), in rounds of whole units
+    auto segment = [&](int k, int todo) {
         for (;;) {
-            const int excl = wave_excl_scan(todo, lane);
-            const int total = __builtin_amdgcn_readfirstlane(wave_sum(todo));
+            const int incl = wave_incl_sum(todo);
+            const int total = wave_total(incl);
             if (total == 0) break;
             if (fill + total > kList && fill > 0) flush();
-            // whole units that fit (a lane's first unit always fits after a flush)
-            int take = min(todo, max(0, kList - fill - excl));
+            const int excl = incl - todo;
+            int take = min(todo, max(0, kList - fill - excl));   // a lane's first unit fits after a flush
             if (take < todo) take &= ~(kUnit - 1);
             const int nut = (take + kUnit - 1) / kUnit;
-            const int uex = wave_excl_scan(nut, lane);
-            const v4f* src = reinterpret_cast<const v4f*>(D.src) + (k0 + from);
-            v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]) + (fill + excl);
-            int u = 0;
-            for (; u + 8 <= take; u += 8) {
-                const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
-                const v4f a4 = src[u + 4], a5 = src[u + 5], a6 = src[u + 6], a7 = src[u + 7];
-                dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
-                dst[u + 4] = a4; dst[u + 5] = a5; dst[u + 6] = a6; dst[u + 7] = a7;
-            }
-            if (u < take) {
-                const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
-                dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
-            }
-            for (int j = 0; j < nut; ++j) uends[wv][nu + uex + j] = fill + excl + min(kUnit * (j + 1), take);
-            fill += __builtin_amdgcn_readfirstlane(wave_sum(take));
-            nu += __builtin_amdgcn_readfirstlane(wave_sum(nut));
-            from += take;
+            const int uincl = wave_incl_sum(nut);
+            copy(k, take, fill + excl, nu + uincl - nut);
+            fill += wave_total(wave_incl_sum(take));
+            nu += wave_total(uincl);
+            k += take;
             todo -= take;
+        }
+    };
+
+    constexpr int kPer = 8;   // candidates per lane kept in registers
+    if (ncand <= 64 * kPer) {
+        int n8[kPer], k8[kPer];
+        int tl = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            n8[j] = brick(64 * j + lane, k8[j]);
+            tl += n8[j];
+        }
+        const int T = wave_total(wave_incl_sum(tl));
+        if (s == 0 && lane == 0) D.tgrp[g] = max(1, T);
+        lo = int(((long long)s * T + S - 1) / S);
+        hi = int(((long long)(s + 1) * T + S - 1) / S);
+        int fr[kPer], td[kPer];
+        int pos = 0, mine = 0, munits = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int incl = wave_incl_sum(n8[j]);
+            td[j] = owned(n8[j], pos + incl - n8[j], fr[j]);
+            pos += wave_total(incl);
+            mine += td[j];
+            munits += (td[j] + kUnit - 1) / kUnit;
+        }
+        const int mincl = wave_incl_sum(mine);
+        if (wave_total(mincl) <= kList) {   // the whole chunk in one gather
+            int o = mincl - mine;
+            int uo = wave_incl_sum(munits) - munits;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                if (td[j] > 0) copy(k8[j] + fr[j], td[j], o, uo);
+                o += td[j];
+                uo += (td[j] + kUnit - 1) / kUnit;
+            }
+            fill = wave_total(mincl);
+            nu = wave_total(wave_incl_sum(munits));
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) segment(k8[j] + fr[j], td[j]);
+        }
+    } else {
+        // large boxes: T first, then the candidates again 64 at a time
+        int tl = 0;
+        for (int cb = 0; cb < ncand; cb += 64) {
+            int k0;
+            tl += brick(cb + lane, k0);
+        }
+        const int T = wave_total(wave_incl_sum(tl));
+        if (s == 0 && lane == 0) D.tgrp[g] = max(1, T);
+        lo = int(((long long)s * T + S - 1) / S);
+        hi = int(((long long)(s + 1) * T + S - 1) / S);
+        int pos = 0;
+        for (int cb = 0; cb < ncand && pos < hi; cb += 64) {
+            int k0, from;
+            const int n = brick(cb + lane, k0);
+            const int incl = wave_incl_sum(n);
+            const int todo = owned(n, pos + incl - n, from);
+            pos += wave_total(incl);
+            segment(k0 + from, todo);
         }
     }
     flush();
-    if (valid) reinterpret_cast<long long*>(D.part)[s * D.ld + i] = acc64;
+    if (valid && acc64 != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&D.isum[i]), (unsigned long long)acc64);
 }
 
 // ===========================================================================
@@ -721,19 +842,7 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     double de = 0.0;
     if (valid && D.coulomb) {
         double sde = 0.0;   // the S chunk partials, in chunk order (deterministic), 16 loads at a time
-        if (D.cull) {   // fixed-point partials of cull_de_kernel: exact integer sum
-            const long long* ip = reinterpret_cast<const long long*>(D.part);
-            long long tot = 0;
-            for (int q0 = 0; q0 < D.S; q0 += 16) {
-                long long v[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) v[u] = ip[min(q0 + u, D.S - 1) * D.ld + i];
-#pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    if (q0 + u < D.S) tot += v[u];
-            }
-            sde = double(tot) * kFixInv;
-        }
+        if (D.cull) sde = double(D.isum[i]) * kFixInv;   // cull_de_kernel's exact fixed-point total
         for (int q0 = 0; q0 < (D.cull ? 0 : D.S); q0 += 16) {
             float2 p[16];   // unconditional loads (clamped index) so that all are in flight together
 #pragma unroll
