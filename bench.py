@@ -12,8 +12,10 @@ conflicts, removals, injection) on the device, with all state resident in HBM.
 
 Prints ONE JSON line.  ``value`` is KMC steps/s; ``pair_evals_per_s`` counts
 2 N (N + Nd) pair evaluations per step (the reference's all-pairs loop).
-``roofline`` is for the dominant kernel (de_kernel), FP32 VALU bound:
-11 flops per in-cutoff pair term, terms counted exactly on the timed state.
+``roofline`` is for the dominant kernel (the pair-sum kernel: cull_de_kernel
+on the culled path, de_kernel on the all-pairs path), FP32 VALU bound: 11 flops
+per in-cutoff pair term, terms counted exactly on the timed state, over the
+kernel's own event-timed duration.
 ``map`` times the full-lattice potential map (256x256x64, 50,000 charges).
 ``batch`` advances 256 independent replicas of uneven size together.
 ``surrogate`` trains the learned map surrogate on snapshots of the solar run
@@ -308,11 +310,15 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary, or None."""
+def measured_traffic(kernel, workload="solar"):
+    """HBM bytes per launch of a kernel from the committed counter summary (measured on the
+    solar workload), or None."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
-            return json.load(f)["kernels"][kernel]["bytes_per_launch"]
+            t = json.load(f)
+        if t.get("workload", "solar") != workload:
+            return None
+        return t["kernels"][kernel]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -358,6 +364,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", choices=("solar", "transistor"), default="solar")
+    ap.add_argument("--de-path", choices=("auto", "pairs", "culled"), default="auto",
+                    help="dE path of the stepped simulation (auto: culled when it pays)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -389,7 +397,7 @@ def main():
     p, e, h, traps, pot = workload
     log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(e)} e + {len(h)} h, "
         f"cutoff {p.electrostatic_cutoff}")
-    sim = L.Simulation(p, electrons=e, holes=h, potential=pot)
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=args.de_path)
     sim.step(args.warmup)
     sim.ctx.sync()
     # exact in-cutoff term count on the state entering the timed region
@@ -402,8 +410,10 @@ def main():
                             sites)
     log(f"timed state: N={n_carriers}, in-cutoff terms per step {terms:.4e}")
 
+    path = sim.de_path
+    kname = "cull_de_kernel" if path == "culled" else "de_kernel"
     t0 = time.perf_counter()
-    de_ms, commit_ms, span_ms = sim.step_timed(args.steps)
+    de_ms, commit_ms, span_ms, pair_ms = sim.step_timed(args.steps)
     wall = time.perf_counter() - t0
     assert sim.device_error() == 0
     st2 = sim.state()
@@ -411,12 +421,14 @@ def main():
     steps_per_s = args.steps / wall
     ms_per_step = wall * 1e3 / args.steps
     de_avg = de_ms / args.steps
+    pair_avg = pair_ms / args.steps
     flops = 11.0 * terms
-    achieved = flops / (de_avg * 1e-3) / 1e12
+    achieved = flops / (pair_avg * 1e-3) / 1e12
     n_avg = 0.5 * (n_carriers + n_end)
     pair_evals_per_s = 2.0 * n_avg * n_avg * steps_per_s
-    log(f"{steps_per_s:.1f} steps/s ({ms_per_step * 1e3:.1f} us/step); de_kernel {de_avg * 1e3:.1f} us, "
-        f"commit {commit_ms / args.steps * 1e3:.1f} us; {achieved:.2f} TFLOP/s")
+    log(f"{steps_per_s:.1f} steps/s ({ms_per_step * 1e3:.1f} us/step); dE phase {de_avg * 1e3:.1f} us "
+        f"({kname} {pair_avg * 1e3:.1f} us), commit {commit_ms / args.steps * 1e3:.1f} us; "
+        f"{achieved:.2f} TFLOP/s")
 
     res = {
         "metric": METRIC,
@@ -437,15 +449,17 @@ def main():
                          if args.workload == "solar" else
                          "transistor 256x256x1, 1310 holes, 1 V bias, cutoff 50"),
             "carriers_start": int(n_carriers), "carriers_end": int(n_end),
+            "de_path": path,
             "parallelism": "single GPU",
         },
         "pair_evals_per_s": pair_evals_per_s,
-        "kernel_us": {"de_kernel": round(de_avg * 1e3, 3),
-                      "commit_kernel": round(commit_ms / args.steps * 1e3, 3),
+        "kernel_us": {"de_phase": round(de_avg * 1e3, 3),
+                      kname: round(pair_avg * 1e3, 3),
+                      "commit_phase": round(commit_ms / args.steps * 1e3, 3),
                       "event_span_per_step": round(span_ms / args.steps * 1e3, 3)},
-        "roofline": {"bound": "valu", "kernel": "de_kernel", "achieved": round(achieved, 4),
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": round(achieved, 4),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": measured_traffic("de_kernel"),
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": measured_traffic(kname, args.workload),
                      "traffic_source": "profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "scaled by a 256 MiB copy probe)",
                      "per_launch": f"11 flops x {terms} in-cutoff terms"},

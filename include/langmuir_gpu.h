@@ -177,11 +177,13 @@ lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
  * (1 or 2). */
 lmg_status lmg_sim_set_de_path(lmg_ctx* ctx, int32_t mode);
 lmg_status lmg_sim_get_de_path(const lmg_ctx* ctx, int32_t* mode);
-/* lmg_sim_step with HIP events around every kernel on the context stream;
- * returns once the steps are done.  kernel_ms[0] = summed dE-kernel time,
- * kernel_ms[1] = summed commit-kernel time, kernel_ms[2] = first-to-last
- * event span (no counterpart in the reference: its only timing is the
- * real:time column of out.dat, writer.cpp:362,376). */
+/* lmg_sim_step with HIP events around the phases of every step on the
+ * context stream; returns once the steps are done.  kernel_ms[0] = summed dE
+ * phase time, kernel_ms[1] = summed commit-phase time, kernel_ms[2] =
+ * first-to-last event span, kernel_ms[3] = summed time of the pair-sum kernel
+ * alone (de_kernel, or cull_de_kernel without the list building before it).
+ * No counterpart in the reference: its only timing is the real:time column of
+ * out.dat, writer.cpp:362,376. */
 lmg_status lmg_sim_step_timed(lmg_ctx* ctx, int32_t n_steps, double* kernel_ms);
 /* Proposed sites and dE of the pending step for every carrier (electrons then
  * holes) without consuming the decision draws: the values

@@ -191,18 +191,24 @@ unsigned long long next_pow2(unsigned long long v) {
 }
 
 // dE pair sums + decision flags + random-stream refill (always launched: with
-// Coulomb off only the flags and the refill run)
-lmg_status sim_launch_de(lmg_ctx* c) {
+// Coulomb off only the flags and the refill run).  `mid`, when given, is
+// recorded before the pair-sum kernel itself (after the culled path's list
+// building).
+lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
     const SimDev& D = c->sim;
     const SimDev* P = c->d_sim;
     if (D.cull) {
         hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream, P);
+        if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
         const dim3 grid((D.items_cap + 3) / 4 + 1);
         if (D.gauss)
             hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P);
         else
             hipLaunchKernelGGL((cull_de_kernel<false>), grid, dim3(kSimBlock), 0, c->stream, P);
-    } else if (!D.coulomb)
+        return launch_check(c, "cull_de_kernel");
+    }
+    if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
+    if (!D.coulomb)
         hipLaunchKernelGGL((de_kernel<false, false>), dim3(D.nblk + 1, 1), dim3(kSimBlock), 0, c->stream, P);
     else if (D.gauss)
         hipLaunchKernelGGL((de_kernel<true, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
@@ -697,33 +703,38 @@ lmg_status lmg_sim_step_timed(lmg_ctx* c, int32_t n, double* kernel_ms) {
     if (n < 1) return fail(c, LMG_ERR_ARG, "step count must be >= 1");
     lmg_status st;
     if ((st = use_device(c)) != LMG_OK) return st;
-    const size_t need = 2 * size_t(n) + 1;
+    const size_t need = 3 * size_t(n) + 1;
     while (c->events.size() < need) {
         hipEvent_t e;
         HIPCHK(c, hipEventCreate(&e));
         c->events.push_back(e);
     }
+    // per step: [3i] dE phase starts, [3i + 1] pair-sum kernel starts,
+    // [3i + 2] dE phase ends (commit starts), [3i + 3] commit ends
     HIPCHK(c, hipEventRecord(c->events[0], c->stream));
     for (int i = 0; i < n; ++i) {
-        if ((st = sim_launch_de(c)) != LMG_OK) return st;
-        HIPCHK(c, hipEventRecord(c->events[2 * i + 1], c->stream));
+        if ((st = sim_launch_de(c, c->events[3 * i + 1])) != LMG_OK) return st;
+        HIPCHK(c, hipEventRecord(c->events[3 * i + 2], c->stream));
         if ((st = sim_launch_commit(c)) != LMG_OK) return st;
-        HIPCHK(c, hipEventRecord(c->events[2 * i + 2], c->stream));
+        HIPCHK(c, hipEventRecord(c->events[3 * i + 3], c->stream));
     }
-    HIPCHK(c, hipEventSynchronize(c->events[2 * n]));
-    double de = 0.0, cm = 0.0;
+    HIPCHK(c, hipEventSynchronize(c->events[3 * n]));
+    double de = 0.0, cm = 0.0, pk = 0.0;
     for (int i = 0; i < n; ++i) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&a, c->events[2 * i], c->events[2 * i + 1]));
-        HIPCHK(c, hipEventElapsedTime(&b, c->events[2 * i + 1], c->events[2 * i + 2]));
+        float a = 0.f, b = 0.f, p = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->events[3 * i], c->events[3 * i + 2]));
+        HIPCHK(c, hipEventElapsedTime(&p, c->events[3 * i + 1], c->events[3 * i + 2]));
+        HIPCHK(c, hipEventElapsedTime(&b, c->events[3 * i + 2], c->events[3 * i + 3]));
         de += a;
+        pk += p;
         cm += b;
     }
     float span = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&span, c->events[0], c->events[2 * n]));
+    HIPCHK(c, hipEventElapsedTime(&span, c->events[0], c->events[3 * n]));
     kernel_ms[0] = de;
     kernel_ms[1] = cm;
     kernel_ms[2] = span;
+    kernel_ms[3] = pk;
     int32_t code = 0;
     if ((st = lmg_sim_device_error(c, &code)) != LMG_OK) return st;
     if (code) return fail(c, LMG_ERR_DEVICE, "device error " + std::to_string(code));

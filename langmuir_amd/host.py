@@ -607,11 +607,12 @@ class Simulation:
     step = perform_iterations
 
     def step_timed(self, n: int):
-        """n steps with HIP events around each kernel; returns
-        (dE-kernel ms summed, commit-kernel ms summed, event span ms)."""
-        out = (C.c_double * 3)()
+        """n steps with HIP events around each phase; returns (dE phase ms,
+        commit phase ms, event span ms, pair-sum kernel ms), each summed over
+        the steps."""
+        out = (C.c_double * 4)()
         self._chk(_abi.lib().lmg_sim_step_timed(self.ctx.handle, int(n), out))
-        return float(out[0]), float(out[1]), float(out[2])
+        return float(out[0]), float(out[1]), float(out[2]), float(out[3])
 
     def device_error(self) -> int:
         code = C.c_int32(0)
