@@ -44,6 +44,10 @@ typedef int32_t lmg_status;
 #define LMG_IMAGE_LEFT 2  /* left-electrode image, dx = |xi + xj| + 1, sign -1
                              (Potential::coulombImage*, potential.cpp:384-415)          */
 #define LMG_NO_CUTOFF  4  /* ignore the cutoff (map only)                               */
+#define LMG_MAP_PAIRS  8  /* map: every charge at every site (no culling)             */
+#define LMG_MAP_CULLED 16 /* map: only charges in 8^3-site bricks within the cutoff of
+                             each 64-site tile (ignored with LMG_NO_CUTOFF).  Neither
+                             flag: culled when the cutoff is well under the lattice */
 
 typedef struct lmg_ctx lmg_ctx;
 

@@ -32,6 +32,8 @@ LMG_ERR_DEVICE = -6
 LMG_GAUSS = 1
 LMG_IMAGE_LEFT = 2
 LMG_NO_CUTOFF = 4
+LMG_MAP_PAIRS = 8
+LMG_MAP_CULLED = 16
 
 FLUX_NAMES = ("eSL", "eSR", "hSL", "hSR", "xS", "eDL", "eDR", "hDL", "hDR", "xD")
 

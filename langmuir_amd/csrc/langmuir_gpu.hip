@@ -24,6 +24,7 @@
 #include "lmg_device.h"
 #include "lmg_pairsum.h"
 #include "lmg_sim.h"
+#include "lmg_mapcull.h"
 
 using namespace lmg;
 
@@ -55,6 +56,7 @@ struct lmg_ctx {
     std::vector<DevBuf*> owned;
     // scratch for the stateless calls
     DevBuf b_tgt, b_chg, b_part, b_site, b_q, b_tsite, b_out, b_map;
+    DevBuf b_mcnt, b_mstart, b_mslot, b_mtmp, b_mtmpi, b_msrc;   // culled map
     // simulation
     bool sim_loaded = false;
     bool sync_mode = false;
@@ -143,8 +145,69 @@ lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int n
     return launch_check(c, "pair_kernel");
 }
 
+// the culled map pays when the cutoff box of a 64-site tile covers well under
+// the lattice and there is enough work to matter
+bool map_culled(const lmg_ctx* c, int nc, int flags) {
+    if ((flags & LMG_NO_CUTOFF) || nc == 0) return false;
+    if (flags & LMG_MAP_CULLED) return true;
+    if (flags & LMG_MAP_PAIRS) return false;
+    auto frac = [&](int n) { return std::min(1.0, double(2 * c->p.cutoff + 12) / n); };
+    return frac(c->d.X) * frac(c->d.Y) * frac(c->d.Z) < 0.5 && double(c->d.V) * nc >= 1e8;
+}
+
+lmg_status map_cull_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, int flags, float* d_out,
+                         const int32_t* d_run) {
+    lmg_status st;
+    MapBricks m;
+    m.nbx = (c->d.X + kMapBrick - 1) / kMapBrick;
+    m.nby = (c->d.Y + kMapBrick - 1) / kMapBrick;
+    m.nbz = (c->d.Z + kMapBrick - 1) / kMapBrick;
+    m.nb = m.nbx * m.nby * m.nbz;
+    const size_t cap = std::max<size_t>(nc, 1);
+    void* before = c->b_mcnt.p;
+    if ((st = ensure(c, c->b_mcnt, size_t(m.nb) * 4)) || (st = ensure(c, c->b_mstart, size_t(m.nb + 1) * 4)) ||
+        (st = ensure(c, c->b_mslot, size_t(nc) * sizeof(int2))) || (st = ensure(c, c->b_mtmp, cap * sizeof(float4))) ||
+        (st = ensure(c, c->b_mtmpi, cap * 4)) || (st = ensure(c, c->b_msrc, cap * sizeof(float4))))
+        return st;
+    if (c->b_mcnt.p != before) HIPCHK(c, hipMemsetAsync(c->b_mcnt.p, 0, size_t(m.nb) * 4, c->stream));
+    m.cnt = static_cast<int*>(c->b_mcnt.p);
+    m.start = static_cast<int*>(c->b_mstart.p);
+    m.slot = static_cast<int2*>(c->b_mslot.p);
+    m.tmp = static_cast<float4*>(c->b_mtmp.p);
+    m.tmpi = static_cast<int*>(c->b_mtmpi.p);
+    m.src = static_cast<float4*>(c->b_msrc.p);
+    const dim3 cg((nc + 255) / 256);
+    hipLaunchKernelGGL(mapc_count_kernel, cg, dim3(256), 0, c->stream, c->d, m, d_cs, nc, d_run);
+    hipLaunchKernelGGL(mapc_scan_kernel, dim3(1), dim3(1024), 0, c->stream, m, d_run);
+    hipLaunchKernelGGL(mapc_scatter_kernel, cg, dim3(256), 0, c->stream, c->d, m, d_cs, d_cq, nc, d_run);
+    hipLaunchKernelGGL(mapc_sort_kernel, dim3((m.nb + 3) / 4), dim3(256), 0, c->stream, m, d_run);
+    const int flat = c->d.Z < 4 ? 1 : 0;
+    const int zc = c->d.Z >= 16 ? 4 : 1;   // sites per lane along z
+    int ts, tz;
+    map_tile_shape(zc, flat, ts, tz);
+    const long long tiles = (long long)((c->d.X + ts - 1) / ts) * ((c->d.Y + ts - 1) / ts) * ((c->d.Z + tz - 1) / tz);
+    const dim3 mg(unsigned((tiles + 3) / 4));
+    const uint32_t c2m1 = cutoff_bits(c->p.cutoff, false);
+    const float gk = gauss_k(c->p.gaussian_sigma);
+    const float pref = float(c->p.prefactor);
+    const bool gauss = flags & LMG_GAUSS;
+#define LMG_MAP_CULL(G, Z) \
+    hipLaunchKernelGGL((map_cull_kernel<G, Z>), mg, dim3(256), 0, c->stream, c->d, m, c->p.cutoff, c2m1, gk, pref, \
+                       flat, d_run, d_out)
+    if (zc == 4) {
+        if (gauss) LMG_MAP_CULL(true, 4);
+        else LMG_MAP_CULL(false, 4);
+    } else {
+        if (gauss) LMG_MAP_CULL(true, 1);
+        else LMG_MAP_CULL(false, 1);
+    }
+#undef LMG_MAP_CULL
+    return launch_check(c, "map_cull_kernel");
+}
+
 lmg_status map_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, int flags, float* d_out,
                     const int32_t* d_run = nullptr) {
+    if (map_culled(c, nc, flags)) return map_cull_core(c, d_cs, d_cq, nc, flags, d_out, d_run);
     lmg_status st;
     if ((st = ensure(c, c->b_chg, size_t(std::max(nc, 1)) * sizeof(float4))) != LMG_OK) return st;
     float4* chg = static_cast<float4*>(c->b_chg.p);
@@ -298,7 +361,8 @@ void lmg_destroy(lmg_ctx* c) {
     (void)hipSetDevice(c->dev);
     (void)hipStreamSynchronize(c->stream);
     sim_free(c);
-    for (DevBuf* b : {&c->b_tgt, &c->b_chg, &c->b_part, &c->b_site, &c->b_q, &c->b_tsite, &c->b_out, &c->b_map})
+    for (DevBuf* b : {&c->b_tgt, &c->b_chg, &c->b_part, &c->b_site, &c->b_q, &c->b_tsite, &c->b_out, &c->b_map,
+                      &c->b_mcnt, &c->b_mstart, &c->b_mslot, &c->b_mtmp, &c->b_mtmpi, &c->b_msrc})
         if (b->p) (void)hipFree(b->p);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -367,7 +431,7 @@ lmg_status lmg_potential_map_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t*
                                  int32_t flags, float* d_out) {
     if (!c) return LMG_ERR_ARG;
     if (nc < 0 || (nc > 0 && (!d_cs || !d_cq)) || !d_out) return fail(c, LMG_ERR_ARG, "invalid map arguments");
-    if (flags & ~(LMG_GAUSS | LMG_NO_CUTOFF)) return fail(c, LMG_ERR_ARG, "unsupported flags");
+    if (flags & ~(LMG_GAUSS | LMG_NO_CUTOFF | LMG_MAP_PAIRS | LMG_MAP_CULLED)) return fail(c, LMG_ERR_ARG, "unsupported flags");
     if ((flags & LMG_GAUSS) && !(c->p.gaussian_sigma > 0))
         return fail(c, LMG_ERR_ARG, "LMG_GAUSS needs gaussian_sigma > 0");
     lmg_status st;
@@ -380,7 +444,7 @@ lmg_status lmg_potential_map_if_dev(lmg_ctx* c, const int32_t* d_cs, const int32
     if (!c) return LMG_ERR_ARG;
     if (nc < 0 || (nc > 0 && (!d_cs || !d_cq)) || !d_out || !d_run)
         return fail(c, LMG_ERR_ARG, "invalid map arguments");
-    if (flags & ~(LMG_GAUSS | LMG_NO_CUTOFF)) return fail(c, LMG_ERR_ARG, "unsupported flags");
+    if (flags & ~(LMG_GAUSS | LMG_NO_CUTOFF | LMG_MAP_PAIRS | LMG_MAP_CULLED)) return fail(c, LMG_ERR_ARG, "unsupported flags");
     if ((flags & LMG_GAUSS) && !(c->p.gaussian_sigma > 0))
         return fail(c, LMG_ERR_ARG, "LMG_GAUSS needs gaussian_sigma > 0");
     lmg_status st;

@@ -207,4 +207,34 @@ __device__ inline int nbr_at(const Dims& d, int h, int site, int dl, int dr, int
     return r;
 }
 
+// Wave-wide prefix sums and reductions with DPP moves (no LDS traffic):
+// row_shr:1,2,4,8 scan each row of 16 lanes, row_bcast:15 and row_bcast:31
+// carry the row totals into the rows above.  Lanes a move has no source for
+// (and rows its row mask leaves out) take the identity.
+template <class Op>
+__device__ __forceinline__ int dpp_scan(int x, int id, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));   // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));   // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));   // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));   // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));   // row_bcast:15 -> rows 1, 3
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));   // row_bcast:31 -> rows 2, 3
+    return x;
+}
+__device__ __forceinline__ int wave_incl_sum(int x) {
+    return dpp_scan(x, 0, [](int a, int b) { return a + b; });
+}
+// the total, wave-uniform (lane 63 of the inclusive scan)
+__device__ __forceinline__ int wave_total(int incl) { return __builtin_amdgcn_readlane(incl, 63); }
+__device__ __forceinline__ float wave_fmin(float v) {
+    const int r = dpp_scan(__float_as_int(v), __float_as_int(3.0e38f),
+                           [](int a, int b) { return __float_as_int(fminf(__int_as_float(a), __int_as_float(b))); });
+    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
+}
+__device__ __forceinline__ float wave_fmax(float v) {
+    const int r = dpp_scan(__float_as_int(v), __float_as_int(-3.0e38f),
+                           [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); });
+    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
+}
+
 }  // namespace lmg

@@ -67,7 +67,7 @@ struct SimDev {
     int* scnt;                    // carriers per brick (cull_mask_kernel; cleared by decide_kernel)
     int* tcnt;                    // targets per brick (same)
     int2* tpos;                   // per target: its brick, its slot in the brick
-    int* sstart;                  // [nbrick + 1] charge list offsets, padded to multiples of 4
+    int* sstart;                  // [nbrick + 1] charge list offsets
     int* tstart;                  // [nbrick] first target slot of each brick (bricks in Morton order)
     const int* bperm;             // bricks in Morton order (static)
     const int* dstart;            // [nbrick + 1] static charged-defect list offsets
@@ -298,8 +298,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 // The lattice is cut into bricks of 8 x 8 x 8 sites (partial at the far faces).
 // Every step the charges are listed brick by brick in a fixed order -- in a
 // brick: electrons, holes (each by site, from occupancy bitmasks), then the
-// static charged defects -- and each brick's list is padded with zero charges
-// to a multiple of 4.  Targets are grouped 64 to a wave in Morton order of
+// static charged defects.  Targets are grouped 64 to a wave in Morton order of
 // their bricks, so a wave's current and proposed sites fill a small box.
 //
 //   cull_mark (propose_kernel, or cull_mark_kernel after serial proposals):
@@ -358,8 +357,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_mark_kernel(const SimDev* __re
     cull_mark(D, i, isE, isE ? D.e_site[cur][i] : D.h_site[cur][i - ne]);
 }
 
-// Offsets of the charge lists (linear brick order, counts padded to multiples
-// of 4; sst[0 .. nb]) and of the targets (bricks in Morton order; tst[brick])
+// Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the targets (bricks in Morton order; tst[brick])
 // by one workgroup of kThreads; thread t scans a contiguous range of bricks,
 // its first kPer counts loaded together.
 template <int kThreads>
@@ -367,7 +365,7 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
     constexpr int kPer = 8;
     const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
     const int b0 = min(nb, int(threadIdx.x) * per), b1 = min(nb, b0 + per);
-    auto scount = [&](int b) { return (D.scnt[b] + D.dstart[b + 1] - D.dstart[b] + 3) & ~3; };
+    auto scount = [&](int b) { return D.scnt[b] + D.dstart[b + 1] - D.dstart[b]; };
     // unconditional loads (clamped index) so that all are in flight together
     int bp[kPer], sc[kPer], tc[kPer];
     int srun = 0, trun = 0;
@@ -504,7 +502,8 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
     }
     const int d0 = D.dstart[b], nd = D.dstart[b + 1] - d0;
     for (int u = lane; u < nd && base + neh + u < end; u += 64) D.src[base + neh + u] = D.dsrc[d0 + u];
-    // zero charges far away pad the list (their terms are exactly 0)
+    // slots left when two carriers of one species share a site (flagged above):
+    // zero charges far away (their terms are exactly 0)
     for (int u = base + neh + nd + lane; u < end; u += 64) D.src[u] = make_float4(-1e4f, -1e4f, -1e4f, 0.f);
 }
 
@@ -549,36 +548,6 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
-// Wave-wide prefix sums and reductions with DPP moves (no LDS traffic):
-// row_shr:1,2,4,8 scan each row of 16 lanes, row_bcast:15 and row_bcast:31
-// carry the row totals into the rows above.  Lanes a move has no source for
-// (and rows its row mask leaves out) take the identity.
-template <class Op>
-__device__ __forceinline__ int dpp_scan(int x, int id, Op op) {
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));   // row_shr:1
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));   // row_shr:2
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));   // row_shr:4
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));   // row_shr:8
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));   // row_bcast:15 -> rows 1, 3
-    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));   // row_bcast:31 -> rows 2, 3
-    return x;
-}
-__device__ __forceinline__ int wave_incl_sum(int x) {
-    return dpp_scan(x, 0, [](int a, int b) { return a + b; });
-}
-// the total, wave-uniform (lane 63 of the inclusive scan)
-__device__ __forceinline__ int wave_total(int incl) { return __builtin_amdgcn_readlane(incl, 63); }
-__device__ __forceinline__ float wave_fmin(float v) {
-    const int r = dpp_scan(__float_as_int(v), __float_as_int(3.0e38f),
-                           [](int a, int b) { return __float_as_int(fminf(__int_as_float(a), __int_as_float(b))); });
-    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
-}
-__device__ __forceinline__ float wave_fmax(float v) {
-    const int r = dpp_scan(__float_as_int(v), __float_as_int(-3.0e38f),
-                           [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); });
-    return __int_as_float(__builtin_amdgcn_readlane(r, 63));
-}
-
 constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
@@ -608,7 +577,7 @@ template <bool kGauss>
 __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = *Dp;
     __shared__ float4 lists[4][kList];
-    __shared__ int uends[4][kList / 4];
+    __shared__ int uends[4][kList];   // a unit holds at least one slot
     if (blockIdx.x == gridDim.x - 1) {
         mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
@@ -682,12 +651,14 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         for (int q = 0; q < nu; ++q) {
             const int e = __builtin_amdgcn_readfirstlane(uends[wv][q]);
             float acc = 0.f;
-            for (; t < e; t += 4) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float4 qc = lists[wv][t + j];
-                    acc = fmaf(qc.w, de_term<kGauss>(p1, qc, hx, hy, hz, hh, c2m1, gk), acc);
-                }
+            for (; t + 2 <= e; t += 2) {
+                const float4 q0 = lists[wv][t], q1 = lists[wv][t + 1];
+                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, hx, hy, hz, hh, c2m1, gk), acc);
+                acc = fmaf(q1.w, de_term<kGauss>(p1, q1, hx, hy, hz, hh, c2m1, gk), acc);
+            }
+            if (t < e) {
+                const float4 q0 = lists[wv][t++];
+                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, hx, hy, hz, hh, c2m1, gk), acc);
             }
             acc64 += (long long)(acc * kFix);
         }
@@ -707,10 +678,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
             dst[u + 4] = a4; dst[u + 5] = a5; dst[u + 6] = a6; dst[u + 7] = a7;
         }
-        if (u < take) {
-            const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
-            dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
-        }
+        for (; u < take; ++u) dst[u] = src[u];
         for (int j = 0; j * kUnit < take; ++j) uends[wv][uo + j] = o + min(kUnit * (j + 1), take);
     };
     // one brick segment per lane, [k, k + todo), in rounds of whole units

@@ -98,26 +98,35 @@ def test_gpu_helper_contract(gpu, oracle):
         assert helper.get_output_host_future(ident) == pytest.approx(ref_fut[ident], rel=1e-5, abs=1e-6)
 
 
-@pytest.mark.parametrize("dims,n,cutoff,flags,sigma", [
-    ((33, 17, 9), 400, 6, 0, 0.0),
-    ((64, 64, 1), 1310, 50, 0, 0.0),
-    ((24, 24, 24), 900, 0, 4, 0.0),     # LMG_NO_CUTOFF
-    ((16, 16, 16), 300, 5, 1, 1.2),     # gauss
+MAP_PATHS = {"pairs": 8, "culled": 16}   # LMG_MAP_PAIRS, LMG_MAP_CULLED
+
+
+@pytest.mark.parametrize("path", sorted(MAP_PATHS))
+@pytest.mark.parametrize("dims,n,cutoff,flags,sigma,dup", [
+    ((33, 17, 9), 400, 6, 0, 0.0, False),      # lattice sides not multiples of the 8-site brick
+    ((64, 64, 1), 1310, 50, 0, 0.0, False),    # flat: 8 x 8 x 1 tiles
+    ((24, 24, 24), 900, 0, 4, 0.0, False),     # LMG_NO_CUTOFF (no culling)
+    ((16, 16, 16), 300, 5, 1, 1.2, False),     # gauss
+    ((40, 24, 16), 700, 9, 0, 0.0, True),      # charges sharing sites
+    ((16, 16, 16), 2600, 7, 0, 0.0, True),     # dense bricks: lists of several 64-slot units
 ])
-def test_potential_map_matches_oracle(gpu, oracle, dims, n, cutoff, flags, sigma):
+def test_potential_map_matches_oracle(gpu, oracle, dims, n, cutoff, flags, sigma, dup, path):
     rng = np.random.default_rng(n)
     V = dims[0] * dims[1] * dims[2]
-    cs = rng.choice(V, n, replace=False).astype(np.int32)
+    cs = rng.choice(V, n, replace=dup).astype(np.int32)
     cq = rng.choice([-1, 1], n).astype(np.int32)
     ctx = gpu.Context(dims, cutoff=max(cutoff, 1), gaussian_sigma=sigma)
-    got = ctx.potential_map(cs, cq, flags).astype(np.float64)
+    got = ctx.potential_map(cs, cq, flags | MAP_PATHS[path]).astype(np.float64)
     cut = cutoff if not (flags & 4) else 0
     ref = oracle.potential_map(dims, cut, cs, cq, flags=flags & 1, sigma=sigma, threads=8)
     sabs = oracle.potential_map(dims, cut, cs, np.abs(cq), flags=flags & 1, sigma=sigma, threads=8)
     tol_check(got, ref, sabs)
+    if path == "culled":   # the brick lists are rebuilt in input order on every call: same bits
+        again = ctx.potential_map(cs, cq, flags | MAP_PATHS[path]).astype(np.float64)
+        assert np.array_equal(got, again)
 
 
-@pytest.mark.parametrize("cutoff,flags", [(50, 0), (50, 4)])
+@pytest.mark.parametrize("cutoff,flags", [(50, 0), (50, 8), (50, 4)])   # automatic (culled), all pairs, no cutoff
 def test_potential_map_full_size_sampled(gpu, oracle, cutoff, flags):
     """256x256x64 lattice, 50,000 charges +/-1: error statistics on 4096 sampled sites."""
     dims = (256, 256, 64)
