@@ -172,6 +172,10 @@ lmg_status lmg_sim_store(lmg_ctx* ctx, lmg_sim_state* state);
  * the context is in synchronous mode (lmg_sim_set_sync). */
 lmg_status lmg_sim_step(lmg_ctx* ctx, int32_t n_steps);
 lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
+/* lmg_sim_step replays one captured step (a HIP graph of its kernels, made at
+ * the first step after a load) when on (the default), or launches the kernels
+ * one by one.  Same results either way.  No counterpart in the reference. */
+lmg_status lmg_sim_set_graphs(lmg_ctx* ctx, int32_t on);
 /* dE path for the following lmg_sim_load: 0 = automatic (culled when a group
  * of targets plus the cutoff covers well under the lattice and the pair count
  * is large), 1 = all pairs, 2 = culled (8^3-site brick lists).  Both give the
@@ -216,6 +220,8 @@ void lmg_batch_destroy(lmg_batch* batch);
 const char* lmg_batch_last_error(const lmg_batch* batch);
 int32_t lmg_batch_size(const lmg_batch* batch);
 lmg_status lmg_batch_step(lmg_batch* batch, int32_t n_steps);
+/* as lmg_sim_set_graphs, for the batch step (graph replay is the default) */
+lmg_status lmg_batch_set_graphs(lmg_batch* batch, int32_t on);
 /* n steps timed with HIP events on the batch stream; *ms = event span */
 lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
 lmg_status lmg_batch_sync(lmg_batch* batch);

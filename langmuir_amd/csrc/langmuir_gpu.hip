@@ -61,6 +61,8 @@ struct lmg_ctx {
     bool sim_loaded = false;
     bool sync_mode = false;
     int de_mode = 0;   // lmg_sim_set_de_path
+    bool use_graphs = true;             // lmg_sim_set_graphs
+    hipGraphExec_t step_exec = nullptr; // one step, captured at the first lmg_sim_step after a load
     lmg_sim_config cfg{};
     SimDev sim{};
     SimDev* d_sim = nullptr;   // device copy the kernels read
@@ -231,6 +233,11 @@ lmg_status map_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc
 }
 
 void sim_free(lmg_ctx* c) {
+    if (c->step_exec) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(c->step_exec);
+        c->step_exec = nullptr;
+    }
     for (void* p : c->sim_allocs) (void)hipFree(p);
     c->sim_allocs.clear();
     c->sim_loaded = false;
@@ -307,6 +314,39 @@ lmg_status sim_launch_commit(lmg_ctx* c) {
     hipLaunchKernelGGL(tick_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, P);
     hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P, 0);
     return sim_launch_propose(c);
+}
+
+// One KMC step as a HIP graph: every kernel reads the device descriptor, so
+// the captured launches stay valid until the next load (sim_free drops it).
+// The step records kernels only (no memset or copy nodes).
+template <class F>
+lmg_status capture_step(hipStream_t st, hipGraphExec_t* exec, F&& enqueue, std::string& err) {
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+        const lmg_status s = enqueue();
+        e = hipStreamEndCapture(st, &graph);
+        if (s != LMG_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return s;
+        }
+    }
+    if (e == hipSuccess) {
+        e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+    }
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        err = std::string("step graph: ") + hipGetErrorString(e);
+        return LMG_ERR_HIP;
+    }
+    return LMG_OK;
+}
+
+lmg_status sim_launch_step(lmg_ctx* c) {
+    lmg_status st;
+    if ((st = sim_launch_de(c)) != LMG_OK) return st;
+    return sim_launch_commit(c);
 }
 
 }  // namespace
@@ -722,6 +762,12 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     return LMG_OK;
 }
 
+lmg_status lmg_sim_set_graphs(lmg_ctx* c, int32_t on) {
+    if (!c) return LMG_ERR_ARG;
+    c->use_graphs = on != 0;
+    return LMG_OK;
+}
+
 lmg_status lmg_sim_set_de_path(lmg_ctx* c, int32_t mode) {
     if (!c) return LMG_ERR_ARG;
     if (mode < 0 || mode > 2) return fail(c, LMG_ERR_ARG, "dE path must be 0 (automatic), 1 (all pairs) or 2 (culled)");
@@ -748,9 +794,16 @@ lmg_status lmg_sim_step(lmg_ctx* c, int32_t n) {
     if (n < 0) return fail(c, LMG_ERR_ARG, "negative step count");
     lmg_status st;
     if ((st = use_device(c)) != LMG_OK) return st;
-    for (int i = 0; i < n; ++i) {
-        if ((st = sim_launch_de(c)) != LMG_OK) return st;
-        if ((st = sim_launch_commit(c)) != LMG_OK) return st;
+    if (c->use_graphs && n > 0) {
+        if (!c->step_exec) {
+            std::string err;
+            if ((st = capture_step(c->stream, &c->step_exec, [&] { return sim_launch_step(c); }, err)) != LMG_OK)
+                return err.empty() ? st : fail(c, st, err);
+        }
+        for (int i = 0; i < n; ++i) HIPCHK(c, hipGraphLaunch(c->step_exec, c->stream));
+    } else {
+        for (int i = 0; i < n; ++i)
+            if ((st = sim_launch_step(c)) != LMG_OK) return st;
     }
     if (c->sync_mode) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -882,6 +935,8 @@ struct lmg_batch {
     SimDev* d_sims = nullptr;
     int max_nblk = 0, max_S = 0;
     bool coulomb = false, gauss = false, any_uniform = false, any_recomb = false;
+    bool use_graphs = true;
+    hipGraphExec_t step_exec = nullptr;   // one step of every replica, captured at the first step
     std::vector<hipEvent_t> events;
 };
 
@@ -923,6 +978,24 @@ lmg_status batch_launch_step(lmg_batch* b) {
     return LMG_OK;
 }
 
+// n steps of every replica: graph replays of one captured step, or kernel by kernel
+lmg_status batch_steps(lmg_batch* b, int n) {
+    if (b->use_graphs && n > 0) {
+        if (!b->step_exec) {
+            std::string err;
+            const lmg_status st = capture_step(b->stream, &b->step_exec, [&] { return batch_launch_step(b); }, err);
+            if (st != LMG_OK) return err.empty() ? st : bfail(b, st, err);
+        }
+        for (int i = 0; i < n; ++i) BHIPCHK(b, hipGraphLaunch(b->step_exec, b->stream));
+        return LMG_OK;
+    }
+    for (int i = 0; i < n; ++i) {
+        const lmg_status st = batch_launch_step(b);
+        if (st != LMG_OK) return st;
+    }
+    return LMG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -933,6 +1006,7 @@ void lmg_batch_destroy(lmg_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->dev);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->step_exec) (void)hipGraphExecDestroy(b->step_exec);
     for (lmg_ctx* c : b->reps) lmg_destroy(c);
     if (b->d_sims) (void)hipFree(b->d_sims);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
@@ -1013,10 +1087,12 @@ lmg_status lmg_batch_step(lmg_batch* b, int32_t n) {
     if (!b) return LMG_ERR_ARG;
     if (n < 0) return bfail(b, LMG_ERR_ARG, "negative step count");
     BHIPCHK(b, hipSetDevice(b->dev));
-    for (int i = 0; i < n; ++i) {
-        lmg_status st = batch_launch_step(b);
-        if (st != LMG_OK) return st;
-    }
+    return batch_steps(b, n);
+}
+
+lmg_status lmg_batch_set_graphs(lmg_batch* b, int32_t on) {
+    if (!b) return LMG_ERR_ARG;
+    b->use_graphs = on != 0;
     return LMG_OK;
 }
 
@@ -1029,10 +1105,8 @@ lmg_status lmg_batch_step_timed(lmg_batch* b, int32_t n, double* kernel_ms) {
         b->events.push_back(e);
     }
     BHIPCHK(b, hipEventRecord(b->events[0], b->stream));
-    for (int i = 0; i < n; ++i) {
-        lmg_status st = batch_launch_step(b);
-        if (st != LMG_OK) return st;
-    }
+    const lmg_status st = batch_steps(b, n);
+    if (st != LMG_OK) return st;
     BHIPCHK(b, hipEventRecord(b->events[1], b->stream));
     BHIPCHK(b, hipEventSynchronize(b->events[1]));
     float ms = 0.f;

@@ -542,6 +542,11 @@ class Batch:
     def sync(self):
         self._chk(_abi.lib().lmg_batch_sync(self._h))
 
+    def set_graphs(self, on: bool = True):
+        """Replay one captured step of every replica as a HIP graph (default) or launch
+        kernel by kernel."""
+        self._chk(_abi.lib().lmg_batch_set_graphs(self._h, int(on)))
+
     def state(self, replica: int) -> dict:
         st, bufs = _store_buffers(*self.caps[replica])
         self._chk(_abi.lib().lmg_batch_store(self._h, int(replica), C.byref(st)))
@@ -599,6 +604,10 @@ class Simulation:
 
     def set_synchronous(self, on: bool = True):
         self._chk(_abi.lib().lmg_sim_set_sync(self.ctx.handle, int(on)))
+
+    def set_graphs(self, on: bool = True):
+        """Replay one captured step as a HIP graph (default) or launch kernel by kernel."""
+        self._chk(_abi.lib().lmg_sim_set_graphs(self.ctx.handle, int(on)))
 
     def perform_iterations(self, n: int):
         """Simulation::performIterations(n) (simulation.cpp:29-111), enqueued."""

@@ -173,6 +173,27 @@ def test_solar_full_size_invariants(gpu, oracle):
     assert a.device_error() == 0
 
 
+@pytest.mark.parametrize("name,de_path", [("solar3d", "culled"), ("transistor2d", "pairs"), ("defects", "culled")])
+def test_graph_replay_matches_eager(gpu, oracle, name, de_path):
+    """lmg_sim_step replays one captured step as a HIP graph (the default); kernel-by-kernel
+    launches give the same state after every call, with host reads and eager work between
+    the replays."""
+    w, a, sp = make(oracle, gpu, name, de_path)
+    b = gpu_from_oracle(gpu, w, sp, de_path=de_path)
+    a.set_synchronous(False)
+    b.set_graphs(False)
+    for n in (1, 3, 1, 6, 2, 9):
+        a.step(n)
+        b.step(n)
+        sa, sb = a.state(), b.state()
+        for k in ("e_site", "h_site", "e_lifetime", "h_lifetime", "e_pathlength", "h_pathlength",
+                  "rng_window", "flux_attempts", "flux_successes"):
+            assert np.array_equal(sa[k], sb[k]), f"{name} after {n} more steps: {k}"
+        assert sa["rng_consumed"] == sb["rng_consumed"] and sa["current_step"] == sb["current_step"]
+        a.delta_e()   # eager dE-only work between replays
+    assert a.device_error() == 0 and b.device_error() == 0
+
+
 def test_load_rejects_bad_state(gpu):
     sp = gpu.SimulationParameters(grid_x=8, grid_y=8, grid_z=1, coulomb_carriers=True,
                                   electron_percentage=0.0, hole_percentage=0.1,
