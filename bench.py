@@ -216,9 +216,10 @@ def bench_batch(steps=200, warmup=20, n_rep=256):
                event_ms_per_step=round(ms / steps, 4),
                replica_steps_per_s=round(n_rep * steps / wall, 1),
                pair_evals_per_s=pair_evals * steps / wall,
-               carriers_total=int(sum(ns)))
+               carriers_total=int(sum(ns)), graph_replay=True, autotuned=batch.tuning)
     log(f"batch: {out['ms_per_step']} ms/step for {n_rep} replicas -> "
-        f"{out['replica_steps_per_s']:.0f} replica-steps/s")
+        f"{out['replica_steps_per_s']:.0f} replica-steps/s (dE chunks {out['autotuned']['charge_chunks']}, "
+        f"tuned dE kernel {out['autotuned']['de_kernel_us']:.0f} us)")
     batch.close()
     return out
 
@@ -466,6 +467,7 @@ def main():
                          "transistor 256x256x1, 1310 holes, 1 V bias, cutoff 50"),
             "carriers_start": int(n_carriers), "carriers_end": int(n_end),
             "de_path": path,
+            "de_shape": sim.tuning,
             "parallelism": "single GPU",
         },
         "pair_evals_per_s": pair_evals_per_s,

@@ -176,6 +176,15 @@ lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
  * the first step after a load) when on (the default), or launches the kernels
  * one by one.  Same results either way.  No counterpart in the reference. */
 lmg_status lmg_sim_set_graphs(lmg_ctx* ctx, int32_t on);
+/* The ΔE kernel's shape after the load: on the all-pairs path the number of
+ * charge chunks, picked by timing each candidate on the loaded state, and its
+ * measured time in microseconds; on the culled path the number of work items
+ * (sized from the occupancy query) and *kernel_us = -1.  The sums do not
+ * depend on either.  No counterpart in the reference. */
+lmg_status lmg_sim_get_tuning(const lmg_ctx* ctx, int32_t* shape, double* kernel_us);
+/* Fixes the all-pairs ΔE kernel's charge-chunk count for the following loads
+ * (1..128) instead of autotuning it (0, the default). */
+lmg_status lmg_sim_set_de_chunks(lmg_ctx* ctx, int32_t chunks);
 /* dE path for the following lmg_sim_load: 0 = automatic (culled when a group
  * of targets plus the cutoff covers well under the lattice and the pair count
  * is large), 1 = all pairs, 2 = culled (8^3-site brick lists).  Both give the
@@ -222,6 +231,9 @@ int32_t lmg_batch_size(const lmg_batch* batch);
 lmg_status lmg_batch_step(lmg_batch* batch, int32_t n_steps);
 /* as lmg_sim_set_graphs, for the batch step (graph replay is the default) */
 lmg_status lmg_batch_set_graphs(lmg_batch* batch, int32_t on);
+/* the common charge-chunk count of the batch's dE kernel, autotuned at
+ * creation on the loaded replicas, and its time for all replicas (us) */
+lmg_status lmg_batch_get_tuning(const lmg_batch* batch, int32_t* chunks, double* kernel_us);
 /* n steps timed with HIP events on the batch stream; *ms = event span */
 lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
 lmg_status lmg_batch_sync(lmg_batch* batch);

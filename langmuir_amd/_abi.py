@@ -107,6 +107,9 @@ def lib():
     L.lmg_sim_set_de_path.argtypes = [VP, I32]
     L.lmg_sim_set_graphs.argtypes = [VP, I32]
     L.lmg_batch_set_graphs.argtypes = [VP, I32]
+    L.lmg_sim_get_tuning.argtypes = [VP, C.POINTER(I32), C.POINTER(C.c_double)]
+    L.lmg_sim_set_de_chunks.argtypes = [VP, I32]
+    L.lmg_batch_get_tuning.argtypes = [VP, C.POINTER(I32), C.POINTER(C.c_double)]
     L.lmg_sim_get_de_path.argtypes = [VP, C.POINTER(I32)]
     L.lmg_sim_delta_e.argtypes = [VP, VP, VP, VP, VP]
     L.lmg_sim_device_error.argtypes = [VP, PI32]
@@ -147,7 +150,8 @@ EXPORTED = (
     "lmg_create", "lmg_destroy", "lmg_last_error", "lmg_device_count", "lmg_sync", "lmg_stream",
     "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_potential_map", "lmg_potential_map_dev",
     "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_set_de_path",
-    "lmg_sim_set_graphs", "lmg_batch_set_graphs",
+    "lmg_sim_set_graphs", "lmg_batch_set_graphs", "lmg_sim_get_tuning", "lmg_batch_get_tuning",
+    "lmg_sim_set_de_chunks",
     "lmg_sim_get_de_path", "lmg_sim_delta_e",
     "lmg_sim_device_error", "lmg_sim_step_timed",
     "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",

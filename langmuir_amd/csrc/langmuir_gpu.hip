@@ -62,6 +62,10 @@ struct lmg_ctx {
     bool sync_mode = false;
     int de_mode = 0;   // lmg_sim_set_de_path
     bool use_graphs = true;             // lmg_sim_set_graphs
+    bool autotune = true;               // tune de_kernel at load (a batch tunes its replicas together)
+    int forced_chunks = 0;              // lmg_sim_set_de_chunks (0: autotune)
+    int tuned_chunks = 0;               // autotuned charge chunks of de_kernel (all-pairs path)
+    double tuned_us = -1.0;             // its measured dE-kernel time
     hipGraphExec_t step_exec = nullptr; // one step, captured at the first lmg_sim_step after a load
     lmg_sim_config cfg{};
     SimDev sim{};
@@ -314,6 +318,71 @@ lmg_status sim_launch_commit(lmg_ctx* c) {
     hipLaunchKernelGGL(tick_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, P);
     hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P, 0);
     return sim_launch_propose(c);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the all-pairs dE kernel's shape: the number of charge chunks S
+// (workgroups along y per block of 256 targets).  Each candidate is timed on
+// the loaded state (one warm-up, then three launches between HIP events).  A
+// dE launch only rewrites the partials, the decision flags and the refilled
+// part of the random ring, which the next step recomputes, and its sums are
+// exact fixed-point totals of the same 32-charge blocks for every S, so the
+// choice changes no result.
+// ---------------------------------------------------------------------------
+constexpr int kMaxChunks = 128;
+
+template <class F>
+double time_launch_us(hipStream_t st, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30;
+    launch();
+    (void)hipEventRecord(a, st);
+    for (int k = 0; k < 3; ++k) launch();
+    (void)hipEventRecord(b, st);
+    float ms = 0.f;
+    const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess &&
+                    hipGetLastError() == hipSuccess;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ok ? 1e3 * double(ms) / 3.0 : 1e30;
+}
+
+// candidates: powers of two up to kMaxChunks with at least one 32-charge block each
+std::vector<int> chunk_candidates(int n_charges) {
+    std::vector<int> v;
+    const int blocks = std::max(1, (n_charges + 31) / 32);
+    for (int S = 1; S <= kMaxChunks && S <= blocks; S *= 2) v.push_back(S);
+    return v;
+}
+
+lmg_status sim_tune_chunks(lmg_ctx* c) {
+    SimDev& D = c->sim;
+    if (D.cull || !D.coulomb) return LMG_OK;
+    if (c->forced_chunks > 0) {
+        D.S = std::min(c->forced_chunks, kMaxChunks);
+        HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+        c->tuned_chunks = D.S;
+        c->tuned_us = -1.0;
+        return LMG_OK;
+    }
+    if (!c->autotune) return LMG_OK;
+    const int cap = D.cap_e + D.cap_h;
+    double best = 1e30;
+    int best_S = D.S;
+    for (int S : chunk_candidates(cap + D.n_d_eff)) {
+        D.S = S;
+        HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+        const double us = time_launch_us(c->stream, [&] { (void)sim_launch_de(c); });
+        if (us < best) {
+            best = us;
+            best_S = S;
+        }
+    }
+    D.S = best_S;
+    HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+    c->tuned_chunks = best_S;
+    c->tuned_us = best;
+    return launch_check(c, "dE autotune");
 }
 
 // One KMC step as a HIP graph: every kernel reads the device descriptor, so
@@ -642,7 +711,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         (st = sim_alloc(c, D.removed, cap)) || (st = sim_alloc(c, D.flag, cap)) ||
         (st = sim_alloc(c, D.tgt1, cap)) || (st = sim_alloc(c, D.tgt2, cap)) ||
         (st = sim_alloc(c, D.chg, cap + D.n_d_eff)) ||
-        (st = sim_alloc(c, D.part, size_t(std::max(D.S, D.S_pairs)) * D.ld)) ||
+        (st = sim_alloc(c, D.part, size_t(std::max(kMaxChunks, D.S_pairs)) * D.ld)) ||
         (st = sim_alloc(c, D.blk_draw, D.nblk)) || (st = sim_alloc(c, D.blk_keep, 2 * D.nblk)) ||
         (st = sim_alloc(c, D.de, cap)) || (st = sim_alloc(c, D.ring, ring)) || (st = sim_alloc(c, c->ctl, 1)) ||
         (st = sim_alloc(c, c->d_sim, 1)))
@@ -755,10 +824,26 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim, 1);
     if ((st = sim_launch_propose(c)) != LMG_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((st = sim_tune_chunks(c)) != LMG_OK) return st;
     c->sim_loaded = true;
     int32_t code = 0;
     if ((st = lmg_sim_device_error(c, &code)) != LMG_OK) return st;
     if (code) return fail(c, LMG_ERR_DEVICE, "device error " + std::to_string(code) + " while preparing");
+    return LMG_OK;
+}
+
+lmg_status lmg_sim_set_de_chunks(lmg_ctx* c, int32_t chunks) {
+    if (!c) return LMG_ERR_ARG;
+    if (chunks < 0) return fail(c, LMG_ERR_ARG, "negative chunk count");
+    c->forced_chunks = chunks;
+    return LMG_OK;
+}
+
+lmg_status lmg_sim_get_tuning(const lmg_ctx* c, int32_t* shape, double* kernel_us) {
+    if (!c || !shape || !kernel_us) return LMG_ERR_ARG;
+    if (!c->sim_loaded) return LMG_ERR_STATE;
+    *shape = c->sim.cull ? c->sim.items_cap : c->sim.S;
+    *kernel_us = c->sim.cull ? -1.0 : c->tuned_us;
     return LMG_OK;
 }
 
@@ -934,6 +1019,8 @@ struct lmg_batch {
     std::vector<lmg_ctx*> reps;
     SimDev* d_sims = nullptr;
     int max_nblk = 0, max_S = 0;
+    int tuned_S = 0;          // autotuned common charge-chunk count
+    double tuned_us = -1.0;   // its dE-kernel time for all replicas
     bool coulomb = false, gauss = false, any_uniform = false, any_recomb = false;
     bool use_graphs = true;
     hipGraphExec_t step_exec = nullptr;   // one step of every replica, captured at the first step
@@ -1032,6 +1119,7 @@ lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_c
             return st;
         }
         b->reps.push_back(c);
+        c->autotune = false;
         st = lmg_sim_load(c, &configs[r], &states[r]);
         if (st != LMG_OK) {
             bfail(nullptr, st, "replica " + std::to_string(r) + ": " + lmg_last_error(c));
@@ -1054,32 +1142,70 @@ lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_c
         lmg_batch_destroy(b);
         return LMG_ERR_HIP;
     }
-    // the replicas already fill the GPU: split each replica's charges into fewer chunks
-    const int S_common = int(std::max<long long>(1, std::min<long long>(64, (8192 + total_waves - 1) / total_waves)));
+    (void)total_waves;
+    // every replica runs the all-pairs dE kernel of one launch for all replicas; the common
+    // charge-chunk count S (capped per replica by its own) is autotuned on the loaded states
     std::vector<SimDev> host(n);
+    int max_charges = 1;
     for (int r = 0; r < n; ++r) {
         SimDev& D = b->reps[r]->sim;
-        D.cull = 0;   // the batch runs the all-pairs dE kernel over all replicas in one launch
-        D.S = std::max(1, std::min(D.S_pairs, S_common));
-        host[r] = D;
+        D.cull = 0;
         b->max_nblk = std::max(b->max_nblk, D.nblk);
-        b->max_S = std::max(b->max_S, D.S);
         b->any_uniform |= D.uniform != 0;
         b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
-        // the single-context descriptor follows the same chunking
-        if (hipMemcpy(b->reps[r]->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice) != hipSuccess) {
-            bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
-            lmg_batch_destroy(b);
-            return LMG_ERR_HIP;
-        }
+        max_charges = std::max(max_charges, D.cap_e + D.cap_h + D.n_d_eff);
     }
-    if (hipMalloc(&b->d_sims, sizeof(SimDev) * n) != hipSuccess ||
-        hipMemcpy(b->d_sims, host.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMalloc(&b->d_sims, sizeof(SimDev) * n) != hipSuccess) {
         bfail(nullptr, LMG_ERR_HIP, "descriptor array allocation failed");
         lmg_batch_destroy(b);
         return LMG_ERR_HIP;
     }
+    auto apply = [&](int S) -> bool {
+        b->max_S = 1;
+        for (int r = 0; r < n; ++r) {
+            SimDev& D = b->reps[r]->sim;
+            D.S = std::max(1, std::min(D.S_pairs, S));
+            host[r] = D;
+            b->max_S = std::max(b->max_S, D.S);
+            // the single-context descriptor follows the same chunking
+            if (hipMemcpy(b->reps[r]->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice) != hipSuccess) return false;
+        }
+        return hipMemcpy(b->d_sims, host.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    double best = 1e30;
+    int best_S = 1;
+    for (int S : chunk_candidates(std::min(max_charges, 32 * 64))) {   // up to 64 chunks
+        if (!apply(S)) {
+            bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
+            lmg_batch_destroy(b);
+            return LMG_ERR_HIP;
+        }
+        const double us = !b->coulomb ? 0.0 : time_launch_us(b->stream, [&] {
+            const dim3 g_de(b->max_nblk + 1, b->max_S, n);
+            if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, dim3(kSimBlock), 0, b->stream, b->d_sims);
+            else hipLaunchKernelGGL((de_kernel<false, true>), g_de, dim3(kSimBlock), 0, b->stream, b->d_sims);
+        });
+        if (us < best) {
+            best = us;
+            best_S = S;
+        }
+        if (!b->coulomb) break;
+    }
+    if (!apply(best_S)) {
+        bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
+        lmg_batch_destroy(b);
+        return LMG_ERR_HIP;
+    }
+    b->tuned_S = best_S;
+    b->tuned_us = b->coulomb ? best : -1.0;
     *out = b;
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_get_tuning(const lmg_batch* b, int32_t* chunks, double* kernel_us) {
+    if (!b || !chunks || !kernel_us) return LMG_ERR_ARG;
+    *chunks = b->tuned_S;
+    *kernel_us = b->tuned_us;
     return LMG_OK;
 }
 

@@ -102,7 +102,7 @@ struct SimDev {
     float4* tgt1;
     float4* tgt2;
     float4* chg;    // [charged defects | electrons | holes]
-    float2* part;
+    long long* part;              // de_kernel's chunk partials, [S][ld], int64 fixed point
     double* de;
     uint32_t* ring;
     unsigned long long ring_mask;
@@ -232,6 +232,14 @@ __device__ __forceinline__ float de_term(const float4& p1, const float4& q, floa
     return (in1 & in2) ? both : one;
 }
 
+// Fixed point of the pair sums: every FP32 sum of a summation unit (a block of
+// 32 charges here, a 64-slot piece of a brick list in cull_de_kernel) is
+// scaled by 2^40, truncated to int64 and added exactly, so a carrier's total
+// does not depend on how the units are split over workgroups or in what order
+// they are added (at most 2^-40 per unit, about 1e-10 eV per carrier).
+constexpr float kFix = 1099511627776.f;        // 2^40
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+
 template <bool kGauss, bool kPairs>
 __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];     // replica
@@ -258,16 +266,17 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
         if (threadIdx.x == 0) D.blk_draw[blockIdx.x] = cnt;
     }
     if (!kPairs) return;
-    const int S = D.S;
-    const int chunk = (nc + S - 1) / S;
-    const int j0 = blockIdx.y * chunk, j1 = min(nc, j0 + chunk);
+    // chunk s: whole blocks of 32 charges [b0, b1) out of nb32
+    const int S = D.S, nb32 = (nc + 31) / 32;
+    const int j0 = 32 * int((long long)blockIdx.y * nb32 / S);
+    const int j1 = min(nc, 32 * int((long long)(blockIdx.y + 1) * nb32 / S));
     const float4 p1 = i < nt ? D.tgt1[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 p2 = i < nt ? D.tgt2[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
     const float hh = hx * hx + hy * hy + hz * hz;
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
-    float s = 0.f, c = 0.f;
+    long long acc64 = 0;
     for (int base = j0; base < j1; base += 256) {
         const int n = min(256, j1 - base);
         __syncthreads();
@@ -279,16 +288,16 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 #pragma unroll 8
             for (int k = k0; k < k0 + 32; ++k)
                 acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
-            kahan(s, c, acc);
+            acc64 += (long long)(acc * kFix);
         }
-        if (k0 < n) {
+        if (k0 < n) {   // the list's last, partial block
             float acc = 0.f;
             for (int k = k0; k < n; ++k)
                 acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
-            kahan(s, c, acc);
+            acc64 += (long long)(acc * kFix);
         }
     }
-    if (i < nt) D.part[blockIdx.y * D.ld + i] = make_float2(s, -c);
+    if (i < nt) D.part[blockIdx.y * D.ld + i] = acc64;
 }
 
 // ===========================================================================
@@ -551,8 +560,6 @@ __device__ __forceinline__ int wave_sum(int v) {
 constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
-constexpr float kFix = 1099511627776.f;        // 2^40: fixed-point scale of the unit sums
-constexpr double kFixInv = 1.0 / 1099511627776.0;
 
 // grid = ceil(items_cap / 4) + 1 workgroups: wave w of workgroup x takes work
 // item 4x + w (group g, chunk s of S); the last workgroup refills the random
@@ -809,15 +816,21 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     }
     double de = 0.0;
     if (valid && D.coulomb) {
-        double sde = 0.0;   // the S chunk partials, in chunk order (deterministic), 16 loads at a time
-        if (D.cull) sde = double(D.isum[i]) * kFixInv;   // cull_de_kernel's exact fixed-point total
-        for (int q0 = 0; q0 < (D.cull ? 0 : D.S); q0 += 16) {
-            float2 p[16];   // unconditional loads (clamped index) so that all are in flight together
+        double sde;
+        if (D.cull) {
+            sde = double(D.isum[i]) * kFixInv;   // cull_de_kernel's exact fixed-point total
+        } else {   // de_kernel's S fixed-point chunk partials, 16 loads in flight
+            const long long* ip = D.part;
+            long long tot = 0;
+            for (int q0 = 0; q0 < D.S; q0 += 16) {
+                long long v[16];   // unconditional loads (clamped index)
 #pragma unroll
-            for (int u = 0; u < 16; ++u) p[u] = D.part[min(q0 + u, D.S - 1) * D.ld + i];
+                for (int u = 0; u < 16; ++u) v[u] = ip[min(q0 + u, D.S - 1) * D.ld + i];
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (q0 + u < D.S) sde += double(p[u].x) + double(p[u].y);
+                for (int u = 0; u < 16; ++u)
+                    if (q0 + u < D.S) tot += v[u];
+            }
+            sde = double(tot) * kFixInv;
         }
         double bf, bs;   // bindingPotential (chargeagent.cpp:463-481)
         if (isE) {

@@ -547,6 +547,13 @@ class Batch:
         kernel by kernel."""
         self._chk(_abi.lib().lmg_batch_set_graphs(self._h, int(on)))
 
+    @property
+    def tuning(self) -> dict:
+        """The autotuned common charge-chunk count of the dE kernel and its time for all replicas."""
+        s, us = C.c_int32(0), C.c_double(0.0)
+        self._chk(_abi.lib().lmg_batch_get_tuning(self._h, C.byref(s), C.byref(us)))
+        return {"charge_chunks": int(s.value), "de_kernel_us": float(us.value)}
+
     def state(self, replica: int) -> dict:
         st, bufs = _store_buffers(*self.caps[replica])
         self._chk(_abi.lib().lmg_batch_store(self._h, int(replica), C.byref(st)))
@@ -567,7 +574,8 @@ class Simulation:
                  potential: Optional[np.ndarray] = None, rng_window: Optional[np.ndarray] = None,
                  rng_consumed: int = 0, e_lifetime=None, e_pathlength=None, h_lifetime=None,
                  h_pathlength=None, flux_attempts=None, flux_successes=None, current_step: int = 0,
-                 device_id: int = 0, context: Optional[Context] = None, de_path: str = "auto"):
+                 device_id: int = 0, context: Optional[Context] = None, de_path: str = "auto",
+                 de_chunks: int = 0):
         params.check()
         self.params = params
         self.ctx = context or Context.from_parameters(params, device_id)
@@ -575,6 +583,7 @@ class Simulation:
         if de_path not in modes:
             raise ValueError(f"de_path must be one of {sorted(modes)}")
         self._chk(_abi.lib().lmg_sim_set_de_path(self.ctx.handle, modes[de_path]))
+        self._chk(_abi.lib().lmg_sim_set_de_chunks(self.ctx.handle, int(de_chunks)))   # 0: autotune
         self.config = make_config(params)
         self.cap_e = max(self.config.max_electrons, 1)
         self.cap_h = max(self.config.max_holes, 1)
@@ -608,6 +617,16 @@ class Simulation:
     def set_graphs(self, on: bool = True):
         """Replay one captured step as a HIP graph (default) or launch kernel by kernel."""
         self._chk(_abi.lib().lmg_sim_set_graphs(self.ctx.handle, int(on)))
+
+    @property
+    def tuning(self) -> dict:
+        """The dE kernel's shape chosen at load: charge chunks and measured time (all pairs),
+        or work items (culled)."""
+        shape, us = C.c_int32(0), C.c_double(0.0)
+        self._chk(_abi.lib().lmg_sim_get_tuning(self.ctx.handle, C.byref(shape), C.byref(us)))
+        if self.de_path == "culled":
+            return {"path": "culled", "work_items": int(shape.value)}
+        return {"path": "pairs", "charge_chunks": int(shape.value), "de_kernel_us": float(us.value)}
 
     def perform_iterations(self, n: int):
         """Simulation::performIterations(n) (simulation.cpp:29-111), enqueued."""

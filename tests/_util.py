@@ -37,9 +37,9 @@ def oracle_state(w):
                 flux_attempts=a, flux_successes=s, current_step=w.current_step)
 
 
-def gpu_from_oracle(L, w, sp, context=None, de_path="auto"):
+def gpu_from_oracle(L, w, sp, context=None, de_path="auto", de_chunks=0):
     st = oracle_state(w)
-    return L.Simulation(sp, context=context, de_path=de_path, **st)
+    return L.Simulation(sp, context=context, de_path=de_path, de_chunks=de_chunks, **st)
 
 
 def resync(sim, w):

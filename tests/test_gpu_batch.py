@@ -56,6 +56,13 @@ def test_batch_matches_single_contexts_and_oracle(gpu, oracle):
             assert not compare_state(bs, w), f"replica {r} step {t}: differs from the oracle"
 
 
+def test_batch_autotuned_chunks(gpu, oracle):
+    _, batch = build(oracle, gpu)
+    t = batch.tuning
+    assert t["charge_chunks"] >= 1 and t["charge_chunks"] & (t["charge_chunks"] - 1) == 0
+    assert t["de_kernel_us"] > 0
+
+
 def test_batch_determinism_and_invariants(gpu, oracle):
     _, a = build(oracle, gpu)
     _, b = build(oracle, gpu)

@@ -194,6 +194,27 @@ def test_graph_replay_matches_eager(gpu, oracle, name, de_path):
     assert a.device_error() == 0 and b.device_error() == 0
 
 
+@pytest.mark.parametrize("name", ["solar3d", "defects", "gauss3d"])
+def test_all_pairs_de_independent_of_chunking(gpu, oracle, name):
+    """The all-pairs dE kernel adds exact fixed-point sums of the same 32-charge blocks
+    whatever its charge-chunk count, so the autotuner's choice changes no bit."""
+    kw = dict(CONFIGS[name])
+    op, sp = params_pair(**kw)
+    w = oracle.World(op)
+    res = []
+    for chunks in (1, 3, 7, 64, 0):
+        sim = gpu_from_oracle(gpu, w, sp, de_path="pairs", de_chunks=chunks)
+        res.append(sim.delta_e()[2:])
+        t = sim.tuning
+        assert t["path"] == "pairs" and t["charge_chunks"] >= 1
+        if chunks:
+            assert t["charge_chunks"] == chunks
+        else:   # autotuned: a power of two with its time measured
+            assert t["charge_chunks"] & (t["charge_chunks"] - 1) == 0 and t["de_kernel_us"] > 0
+    for de, dh in res[1:]:
+        assert np.array_equal(de, res[0][0]) and np.array_equal(dh, res[0][1])
+
+
 def test_load_rejects_bad_state(gpu):
     sp = gpu.SimulationParameters(grid_x=8, grid_y=8, grid_z=1, coulomb_carriers=True,
                                   electron_percentage=0.0, hole_percentage=0.1,
