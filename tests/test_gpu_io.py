@@ -1,0 +1,93 @@
+"""The langmuir run loop on the device driven by an input file (langmuir_amd.io.run_input).
+
+* outputs: %stub.parm, out.dat rows at every print interval, %stub-%step.coulomb maps
+  every output.coulomb intervals and %stub.chk checkpoints, with the layouts pinned
+  on the CPU in test_io.py;
+* the .coulomb values are the full-lattice map of the carriers at that step, checked
+  against the FP64 oracle map (tolerance as in test_gpu_pairs: 1e-5 relative plus
+  5e-8 of the sum of term magnitudes);
+* resume: a run stopped at a checkpoint and restarted from that file ends in the
+  same state (sites in list order, flux counters, engine window) as the run that
+  went straight through -- [RandomState] carries the engine exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from langmuir_amd import io as L
+
+pytestmark = pytest.mark.gpu
+
+INPUT = """[Parameters]
+simulation.type = transistor
+random.seed = 17
+grid.x = 40
+grid.y = 32
+grid.z = 1
+coulomb.carriers = true
+electron.percentage = 0.0
+hole.percentage = 0.04
+seed.charges = 0.0
+e.source.l.rate = 0.0
+h.source.l.rate = 0.9
+h.drain.r.rate = 0.9
+voltage.left = 1.0
+iterations.print = 10
+iterations.real = {real}
+output.coulomb = 2
+output.step.chk = 1
+output.stub = run
+"""
+
+
+def write_input(path, real):
+    with open(path, "w") as f:
+        f.write(INPUT.format(real=real))
+    return str(path)
+
+
+def same_state(a, b):
+    for k in ("e_site", "h_site", "flux_attempts", "flux_successes", "rng_window"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    assert a["current_step"] == b["current_step"]
+
+
+def test_run_input_outputs(gpu, oracle, tmp_path):
+    print("run_input: 40x32 transistor, 40 steps", flush=True)
+    state, kv, setup = L.run_input(write_input(tmp_path / "in.inp", 40), str(tmp_path / "out"))
+    out = tmp_path / "out"
+    assert sorted(os.listdir(out)) == ["run-20.coulomb", "run-40.coulomb", "run.chk", "run.dat", "run.parm"]
+    head, rows = L.read_flux_table(str(out / "run.dat"))
+    assert rows[:, 0].tolist() == [10, 20, 30, 40]
+    assert rows[-1, 21] == len(state["e_site"]) and rows[-1, 22] == len(state["h_site"])
+    assert np.array_equal(rows[-1, 1:21:2], state["flux_attempts"])
+    ck = L.read_checkpoint(str(out / "run.chk"))
+    assert ck.holes == state["h_site"].tolist() and ck.parameters["current.step"] == 40
+    assert ck.random_state == state["rng_window"].tolist()
+    assert L.read_checkpoint(str(out / "run.parm")).parameters["output.stub"] == "run"
+    # the map at step 40: every site, the carriers of that step
+    s, v = L.read_coulomb(str(out / "run-40.coulomb"))
+    assert sorted(s.tolist()) == list(range(40 * 32))
+    sites = np.concatenate([state["e_site"], state["h_site"]]).astype(np.int32)
+    q = np.concatenate([-np.ones(len(state["e_site"])), np.ones(len(state["h_site"]))]).astype(np.int32)
+    assert len(sites) > 0
+    ref = oracle.potential_map((40, 32, 1), 50, sites, q)
+    sabs = oracle.potential_map((40, 32, 1), 50, sites, np.abs(q))
+    err = np.abs(v - ref[s])
+    assert np.all(err <= 1e-5 * np.abs(ref[s]) + 5e-8 * sabs[s] + 1e-14 * np.max(np.abs(ref))), err.max()
+
+
+def test_checkpoint_resume_is_exact(gpu, tmp_path):
+    print("resume: 20 + 20 steps against 40", flush=True)
+    straight, _, _ = L.run_input(write_input(tmp_path / "a.inp", 40), str(tmp_path / "a"))
+    L.run_input(write_input(tmp_path / "b.inp", 20), str(tmp_path / "b"))
+    ck = L.read_checkpoint(str(tmp_path / "b" / "run.chk"))
+    assert ck.parameters["current.step"] == 20
+    ck.parameters["iterations.real"] = 40
+    L.write_checkpoint(str(tmp_path / "b" / "resume.chk"), ck)
+    resumed, _, _ = L.run_input(str(tmp_path / "b" / "resume.chk"), str(tmp_path / "c"))
+    same_state(resumed, straight)
+    _, rows_a = L.read_flux_table(str(tmp_path / "a" / "run.dat"))
+    _, rows_c = L.read_flux_table(str(tmp_path / "c" / "run.dat"))
+    assert np.array_equal(rows_a[2:, :23], rows_c[:, :23])
