@@ -54,17 +54,11 @@ def log(msg):
 # synthetic inputs
 # ---------------------------------------------------------------------------
 def morphology_traps(dims, sigma=4.0, seed=2024):
-    """Two-phase morphology: uniform noise in [-0.5, 0.5), Gaussian-smoothed
-    (sigma, truncated at 3 sigma), thresholded at its median; returns the site ids
-    of one phase (50 % of the lattice)."""
-    from scipy.ndimage import gaussian_filter
-    X, Y, Z = dims
-    rng = np.random.default_rng(seed)
-    noise = rng.random((Z, Y, X)) - 0.5
-    smooth = gaussian_filter(noise, sigma, mode="constant", truncate=3.0)
-    flat = smooth.ravel()
-    order = np.argsort(flat, kind="stable")
-    return np.sort(order[len(order) // 2:]).astype(np.int32)
+    """Two-phase morphology made the LangmuirPython way (uniform noise in [-0.5, 0.5)
+    convolved with a normalised Gaussian, sigma 4 truncated at 3 sigma, mode 'same'),
+    split at its median; returns the site ids of one phase (50 % of the lattice)."""
+    from langmuir_amd.morphology import two_phase_traps
+    return two_phase_traps(dims, sigma, seed)[0]
 
 
 def solar_workload(seed=2024):

@@ -28,10 +28,10 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .host import SimulationParameters, background_potential
+from .host import SimulationParameters
 
 # (key, type, default, constant) in registration order (keyvalueparser.cpp:15-105;
-# defaults parameters.h:285-390).  Types: "str", "i32", "uint", "real", "bool".
+# defaults parameters.h:285-390).  Types: "str", "i32", "u32", "u64", "real", "bool", "datetime".
 PARAMETER_TABLE: Tuple[Tuple[str, str, object, bool], ...] = (
     ("simulation.type", "str", "transistor", False),
     ("current.step", "u32", 0, False),
@@ -319,6 +319,33 @@ def _open_text(path: str) -> str:
     return raw.decode("latin-1")
 
 
+
+
+def fix_traps(ck: Checkpoint) -> None:
+    """LangmuirPython's CheckPoint.fix_traps (LangmuirPython/langmuir/checkpoint.py:337-388):
+    trap.percentage = (n + 0.25) / V, so that int(trap.percentage * V) is the listed n and
+    World seeds no more traps; seed.percentage = 0; a single shared trap potential moves
+    into trap.potential (a mixed list stays, with trap.potential = 0)."""
+    if not ck.traps:
+        return
+    size = len(ck.traps)
+    if ck.trap_potentials:
+        if len(ck.trap_potentials) != size:
+            raise ValueError("number of traps and potentials do not agree")
+        first = ck.trap_potentials[0]
+        value = first if all(v == first for v in ck.trap_potentials) else None
+    else:
+        value = ck.parameters["trap.potential"]
+    volume = ck.parameters["grid.x"] * ck.parameters["grid.y"] * ck.parameters["grid.z"]
+    if size > volume:
+        raise ValueError(f"number of traps exceeds volume: {size} > {volume}")
+    ck.parameters["trap.percentage"] = (size + 0.25) / volume
+    ck.parameters["seed.percentage"] = 0.0
+    if value is None:
+        ck.parameters["trap.potential"] = 0.0
+    else:
+        ck.trap_potentials = None
+        ck.parameters["trap.potential"] = float(value)
 
 
 def read_checkpoint(path: str) -> Checkpoint:

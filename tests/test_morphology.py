@@ -1,0 +1,104 @@
+"""LangmuirPython's morphology generator (langmuir_amd.morphology) on the CPU.
+
+Pinned by: a brute-force direct 3-D convolution with zero padding (mode 'same');
+the Gaussian closed form of the kernel; numpy's legacy global generator (what
+``numpy.random.random`` gives after ``numpy.random.seed``); hand cases of the
+reference's threshold and map_mesh rules (surface.py:265-294, grid.py:395-403);
+and examples/gyroid.chk, whose 442,368 traps on 96^3 carry trap.percentage =
+0.50000028257 as written by CheckPoint.fix_traps.
+"""
+import numpy as np
+import pytest
+
+from langmuir_amd import io as L
+from langmuir_amd import morphology as M
+
+
+def test_kernel_closed_form():
+    k = M.gaussian_kernel(4.0, 4.0, 4.0)
+    assert k.shape == (25, 25, 25) and abs(k.sum() - 1.0) < 1e-12
+    x = np.arange(-12, 13, dtype=np.float64)
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    ref = np.exp(-r2 / 32.0)
+    assert np.allclose(k, ref / ref.sum(), rtol=1e-12, atol=0)
+    k2 = M.gaussian_kernel(1.0, 2.0, 0.5)
+    assert k2.shape == (7, 13, 4)      # -3s .. 3s in steps of 1 (0.5 -> -1.5, -0.5, 0.5, 1.5)
+
+
+def test_noise_is_numpys_legacy_stream():
+    np.random.seed(123)
+    want = np.random.random((5, 4, 3)) - 0.5
+    assert np.array_equal(M.noise((5, 4, 3), 123), want)
+
+
+def test_convolution_matches_direct_sum():
+    rng = np.random.default_rng(1)
+    f = rng.random((9, 7, 5)) - 0.5
+    k = M.gaussian_kernel(1.0, 1.0, 1.0)      # 7^3, centre (3, 3, 3)
+    got = M.isotropic(f.shape, 1.0, field=f)
+    ref = np.zeros_like(f)
+    for i in range(9):
+        for j in range(7):
+            for l in range(5):
+                acc = 0.0
+                for a in range(-3, 4):
+                    for b in range(-3, 4):
+                        for c in range(-3, 4):
+                            ii, jj, ll = i - a, j - b, l - c
+                            if 0 <= ii < 9 and 0 <= jj < 7 and 0 <= ll < 5:
+                                acc += f[ii, jj, ll] * k[a + 3, b + 3, c + 3]
+                ref[i, j, l] = acc
+    assert np.max(np.abs(got - ref)) < 1e-14
+
+
+def test_threshold_and_map_mesh_rules():
+    a = np.array([-0.4, -0.1, 0.0, 0.2, 0.7, 1.5]).reshape(6, 1, 1)
+    assert M.threshold(a, 0.1).ravel().tolist() == [0, 0, 0, 1, 1, 1]
+    # v < 0: values in (v, 0] are left as they are
+    assert M.threshold(a, -0.2).ravel().tolist() == [0, -0.1, 0, 1, 1, 1]
+    assert M.map_mesh(M.threshold(a, -0.2)).tolist() == [3, 4, 5]
+    mask = np.zeros((3, 2, 2))
+    mask[2, 0, 0] = mask[0, 1, 1] = mask[1, 0, 1] = 1
+    # ndenumerate order (x outermost), site = X (y + Y z) + x
+    assert M.map_mesh(mask).tolist() == [0 + 3 * (1 + 2 * 1), 1 + 3 * (0 + 2 * 1), 2]
+
+
+def test_two_phase_traps_split_at_the_median():
+    traps, image = M.two_phase_traps((24, 20, 8), sigma=2.0, seed=5)
+    assert len(traps) == 24 * 20 * 8 // 2 and len(set(traps.tolist())) == len(traps)
+    flat = np.zeros(24 * 20 * 8)
+    X, Y = 24, 20
+    xi, yi, zi = np.meshgrid(np.arange(24), np.arange(20), np.arange(8), indexing="ij")
+    flat[(X * (yi + Y * zi) + xi).ravel()] = image.ravel()
+    inside = np.zeros(len(flat), bool)
+    inside[traps] = True
+    assert flat[inside].min() > flat[~inside].max()
+
+
+def test_fix_traps_matches_reference_checkpoint():
+    """examples/gyroid.chk: 442368 traps on 96^3 and trap.percentage = 0.50000028257."""
+    ck = M.to_checkpoint(np.arange(442368), (96, 96, 96), trap_potential=0.5)
+    pct = ck.parameters["trap.percentage"]
+    assert f"{pct:.11f}" == "0.50000028257" and int(pct * 96 ** 3) == 442368
+    assert ck.parameters["seed.percentage"] == 0.0 and ck.parameters["trap.potential"] == 0.5
+    ck = L.Checkpoint(traps=[1, 2], trap_potentials=[0.2, 0.3])
+    L.fix_traps(ck)
+    assert ck.trap_potentials == [0.2, 0.3] and ck.parameters["trap.potential"] == 0.0
+    ck = L.Checkpoint(traps=[1, 2], trap_potentials=[0.2])
+    with pytest.raises(ValueError):
+        L.fix_traps(ck)
+
+
+def test_morphology_drives_world_setup():
+    """A generated morphology written as a checkpoint sets up with exactly its traps."""
+    from langmuir_amd import world as W
+    traps, _ = M.two_phase_traps((16, 12, 6), sigma=1.5, seed=9)
+    ck = M.to_checkpoint(traps, (16, 12, 6))
+    ck.parameters["random.seed"] = 3
+    back = L.parse_checkpoint(L.format_checkpoint(L.Checkpoint(
+        parameters=ck.parameters, traps=ck.traps, random_state=W.MT19937(3).window().tolist())))
+    setup = L.setup_of(back)
+    assert setup.traps == traps.tolist()
+    pot = setup.potential
+    base = W.background_potential(setup.params)
+    assert np.array_equal(np.nonzero(pot - base)[0], np.sort(traps))
