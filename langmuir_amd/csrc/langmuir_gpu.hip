@@ -686,16 +686,15 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     }
     if (D.cull) {
         D.cut = c->p.cutoff;
-        D.ngrp = (cap + 63) / 64;
-        // as many chunks per target group as keep every workgroup of
-        // cull_de_kernel resident at once (one wave of workgroups, no tail)
+        D.ngrp = (cap + kCullGroup - 1) / kCullGroup;
         int ncu = 0, per_cu = 0;
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                       &per_cu, reinterpret_cast<const void*>(&cull_de_kernel<false>), kSimBlock, 0));
-        // work items: as many as keep every workgroup resident at once (one
-        // wave of workgroups, no tail), at least one per target group
-        D.items_cap = std::max(4 * (ncu * per_cu - 1), D.ngrp);
+        // work items (workgroups of four chunks of one target group): as many
+        // as keep every workgroup of cull_de_kernel resident at once (one wave
+        // of workgroups, no tail), at least one per group; items_cap counts waves
+        D.items_cap = 4 * std::max(ncu * per_cu - 1, D.ngrp);
     }
 
     for (int b = 0; b < 2; ++b) {

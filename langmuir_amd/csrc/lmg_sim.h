@@ -308,26 +308,22 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 // Every step the charges are listed brick by brick in a fixed order -- in a
 // brick: electrons, holes (each by site, from occupancy bitmasks), then the
 // static charged defects.  Targets are grouped 64 to a wave in Morton order of
-// their bricks, so a wave's current and proposed sites fill a small box.
+// their bricks, so a group's current and proposed sites fill a small box.
 //
 //   cull_mark (propose_kernel, or cull_mark_kernel after serial proposals):
 //     each carrier's bitmask bit, per-brick charge and target counts, and the
 //     carrier's slot among its brick's targets.
 //   propose_fix_kernel (or cull_scan_kernel): the counts become list offsets.
-//   cull_emit_kernel: one wave per brick writes its charge list; the other workgroups set the
-//     decision-draw flags (as de_kernel's chunk-0 workgroups) and put the
-//     targets into their brick's slots.
-//   cull_de_kernel: wave w, chunk s takes the bricks b with b mod S = s inside
-//     the box widened by the cutoff whose distance to the box is below the
-//     cutoff, and sums brick by brick: an FP32 sum over the brick's list, added with Kahan
-//     compensation when nonzero.  A skipped brick holds only charges at
-//     r1, r2 >= c for every target of the wave, whose terms are exactly 0, so
-//     each target's chunk sums are the same bits whatever its wave: the result
-//     is deterministic although the grouping (atomic scatter) is not.
+//   cull_emit_kernel: one wave per brick writes its charge list; the other
+//     workgroups set the decision-draw flags and put the targets into their
+//     brick's slots; the last one writes cull_de_kernel's work items.
+//   cull_de_kernel: the pair sums of every target over the lists of the bricks
+//     near its group (below), as int64 fixed-point totals.
 //   decide_kernel (committing pass) clears the bitmask words and brick
 //     counters each carrier set.
 // ===========================================================================
 constexpr int kBrick = 8;          // brick edge (sites)
+constexpr int kCullGroup = 64;     // targets per group: one per lane of a wave
 constexpr int kMaxCullBricks = 1 << 16;
 
 __device__ __forceinline__ int cull_brick(const SimDev& D, int x, int y, int z) {
@@ -432,7 +428,7 @@ __global__ __launch_bounds__(kOneBlock) void cull_scan_kernel(const SimDev* __re
 // number of item slots left after one item per group; items are (g, chunk,
 // S_g) in group order.  Any S_g gives the same sums: it only balances the work.
 __device__ void cull_items(const SimDev& D, int* wsum) {
-    const int ng = (D.ctl->counts[0] + 63) / 64;
+    const int ng = (D.ctl->counts[0] + kCullGroup - 1) / kCullGroup;
     const int per = (ng + kSimBlock - 1) / kSimBlock;
     const int g0 = min(ng, int(threadIdx.x) * per), g1 = min(ng, g0 + per);
     long long wl = 0;
@@ -443,16 +439,17 @@ __device__ void cull_items(const SimDev& D, int* wsum) {
     if ((threadIdx.x & 63) == 0) wtot[threadIdx.x >> 6] = wl;
     __syncthreads();
     const long long W = max(1ll, wtot[0] + wtot[1] + wtot[2] + wtot[3]);
-    const long long K = max(0, D.items_cap - ng);
-    auto chunks = [&](int g) { return 1 + int((K * D.tgrp[g]) / W); };
+    // workgroups of four chunks: one per group, the rest shared in proportion to T
+    const long long K = max(0, D.items_cap / 4 - ng);
+    auto quads = [&](int g) { return 1 + int((K * D.tgrp[g]) / W); };
     int run = 0;
-    for (int g = g0; g < g1; ++g) run += chunks(g);
+    for (int g = g0; g < g1; ++g) run += quads(g);
     int total;
     int k = wg_scan<kSimBlock>(run, wsum, total);
     for (int g = g0; g < g1; ++g) {
-        const int sg = chunks(g);
-        for (int c = 0; c < sg; ++c) D.items[k + c] = make_int4(g, c, sg, 0);
-        k += sg;
+        const int qg = quads(g);
+        for (int c = 0; c < qg; ++c) D.items[k + c] = make_int4(g, c, qg, 0);
+        k += qg;
     }
     if (threadIdx.x == 0) *D.nitems = total;
 }
@@ -561,14 +558,16 @@ constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
 
-// grid = ceil(items_cap / 4) + 1 workgroups: wave w of workgroup x takes work
-// item 4x + w (group g, chunk s of S); the last workgroup refills the random
-// stream.
+// grid = ceil(items_cap / 4) + 1 workgroups: workgroup x takes work item x
+// (group g, quad q of Q) and its wave w chunk s = 4 q + w of S = 4 Q; the last
+// workgroup refills the random stream.
 //
 // Every wave of group g finds the same work: the bricks of the box widened by
-// the cutoff whose distance to the box is below the cutoff (tested 64 at a
-// time, lane per brick), their lists cut into units (64-slot pieces), T slots
-// in all.  Chunk s takes the units whose first slot p (in candidate order) has
+// the cutoff whose distance to the box is below the cutoff, their lists cut
+// into units (64-slot pieces), T slots in all.  For boxes of up to 512
+// candidate bricks the four waves test them together (two per thread) into a
+// table in LDS; larger boxes are tested by each wave, 64 at a time.  Chunk s
+// takes the units whose first slot p (in candidate order) has
 // floor(p S / T) = s, so every chunk gets T / S slots give or take a unit:
 //   gather: each lane copies its bricks' units of the chunk into the wave's
 //     LDS list (its own 64-byte rows, all lanes at once) at the exclusive scan
@@ -576,10 +575,10 @@ constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's lis
 //   sum: every lane runs over the list (broadcast reads): an FP32 sum per unit,
 //     taken to fixed point (x 2^40, truncated) and added in int64; the chunk's
 //     total goes into the target's isum with a 64-bit integer atomic.
-// The units and their FP32 sums are the same for a target in any group, and
-// the units a group skips hold only terms that are exactly 0 for its targets;
-// integer addition is exact, so each target's total is the same whatever its
-// group, chunk count or order.
+// The units and their FP32 sums are the same for a target in any group,
+// and the units a group skips hold only terms that are exactly 0 for its
+// targets; integer addition is exact, so each target's total is the same
+// whatever its group, chunk count or order.
 template <bool kGauss>
 __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = *Dp;
@@ -591,11 +590,10 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     }
     const int nt = D.ctl->counts[0];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int item = blockIdx.x * 4 + wv;
-    if (item >= *D.nitems) return;   // whole wave; nothing below synchronises the workgroup
-    const int4 it = D.items[item];
-    const int g = it.x, s = it.y, S = it.z;
-    const int r = g * 64 + lane;
+    if (int(blockIdx.x) >= *D.nitems) return;   // whole workgroup
+    const int4 it = D.items[blockIdx.x];
+    const int g = it.x, s = 4 * it.y + wv, S = 4 * it.z;
+    const int r = g * kCullGroup + lane;
     const bool valid = r < nt;
     const int i = valid ? D.torder[r] : 0;
     float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f), p2 = p1;
@@ -709,14 +707,24 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     };
 
     constexpr int kPer = 8;   // candidates per lane kept in registers
-    if (ncand <= 64 * kPer) {
+    if (ncand <= 64 * kPer) {   // the same for the four waves (one group)
+        int* tab_n = &uends[0][0];   // 512 candidates' list lengths and starts, before
+        int* tab_k = &uends[1][0];   // the unit ends take this space
+        for (int ci = threadIdx.x; ci < 64 * kPer; ci += kSimBlock) {
+            int k0;
+            tab_n[ci] = brick(ci, k0);
+            tab_k[ci] = k0;
+        }
+        __syncthreads();
         int n8[kPer], k8[kPer];
         int tl = 0;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            n8[j] = brick(64 * j + lane, k8[j]);
+            n8[j] = tab_n[64 * j + lane];
+            k8[j] = tab_k[64 * j + lane];
             tl += n8[j];
         }
+        __syncthreads();   // every wave holds its candidates before unit ends overwrite the table
         const int T = wave_total(wave_incl_sum(tl));
         if (s == 0 && lane == 0) D.tgrp[g] = max(1, T);
         lo = int(((long long)s * T + S - 1) / S);
