@@ -283,6 +283,26 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
     full_err = float(np.linalg.norm(d_out.download(np.float32, sur.V) - snaps.targets(n_snap - 1))
                      / np.linalg.norm(snaps.targets(n_snap - 1)))
     ms_infer = timed(lambda: sur.infer_dev(d_occ.ptr, 1, d_out.ptr), reps)
+    # the eight held-out snapshots as one batch of gated maps
+    nb = n_snap - n_train
+    lists = []
+    for i in range(n_train, n_snap):
+        o = snaps.occupancy(i).astype(np.int64)
+        si = np.nonzero(o)[0].astype(np.int32)
+        lists.append((si, o[si].astype(np.int32)))
+    counts = [len(a) for a, _ in lists]
+    b_s = L.DeviceBuffer.from_array(np.concatenate([a for a, _ in lists]))
+    b_q = L.DeviceBuffer.from_array(np.concatenate([c for _, c in lists]))
+    b_out = L.DeviceBuffer(4 * sur.V * nb)
+    ms_batch = timed(lambda: sur.map_batch_dev(b_s.ptr, b_q.ptr, counts, b_out.ptr, 1e-3, wait=False), reps)
+    b_err, b_used = sur.last_gates(nb)
+    b_maps = b_out.download(np.float32, sur.V * nb).reshape(nb, sur.V)
+    b_full = [float(np.linalg.norm(b_maps[k] - snaps.targets(n_train + k)) / np.linalg.norm(snaps.targets(n_train + k)))
+              for k in range(nb)]
+    us_per_map = ms_batch * 1e3 / nb
+    log(f"surrogate batch of {nb} gated maps: {ms_batch * 1e3:.1f} us ({us_per_map:.1f} us per map, gate errors <= "
+        f"{b_err.max():.2e}, exact used {int(b_used.sum())}/{nb}, full-map rel RMS <= {max(b_full):.2e}) -> "
+        f"{ms_exact * 1e3 / us_per_map:.1f}x the culled exact map, {ms_pairs * 1e3 / us_per_map:.1f}x all pairs")
     shapes = surrogate_gemm_shapes(dims, sur.padded)
     alg = sum(2.0 * m * n * k * b for m, n, k, b in shapes)
     executed = alg * 3 - 2.0 * shapes[0][0] * shapes[0][1] * shapes[0][2]   # int8 stage: two products
@@ -303,6 +323,12 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
         "exact_map_all_pairs_us": round(ms_pairs * 1e3, 2),
         "rel_rms_heldout_max": max(errs), "gate_error": gate_err, "gate_sites": 1024, "used_exact": used,
         "rel_rms_gated_map": full_err,
+        "batch": {"maps": nb, "gated_batch_us": round(ms_batch * 1e3, 2), "us_per_map": round(us_per_map, 2),
+                  "maps_per_s": round(1e6 / us_per_map, 1),
+                  "speedup_vs_exact": round(ms_exact * 1e3 / us_per_map, 2),
+                  "speedup_vs_exact_all_pairs": round(ms_pairs * 1e3 / us_per_map, 2),
+                  "gate_error_max": float(b_err.max()), "used_exact": int(b_used.sum()),
+                  "rel_rms_max": max(b_full)},
         "weights_stored": 2 * sur.n_weights, "weights_free": free, "padded": list(sur.padded),
         "snapshots": n_snap, "snapshot_stride_steps": stride, "train_snapshots": n_train,
         "train_iterations": iters, "train_s": round(t_train, 3), "loss_first_last": [float(hist[0]), float(hist[-1])],

@@ -299,6 +299,19 @@ lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const 
                                  int32_t* used_exact);
 /* Outcome of the latest lmg_surrogate_map_dev (waits for the stream). */
 lmg_status lmg_surrogate_last_gate(lmg_surrogate* s, double* gate_error, int32_t* used_exact);
+/* A batch of gated maps in one call (1 <= batch <= max_batch, grid_z * batch
+ * <= 65535): map b is that of charges [o_b, o_b + counts[b]) of d_site / d_q,
+ * o_b = counts[0] + ... + counts[b-1] (counts on the host), written to
+ * d_map + b V; each map has its own gate over the same gate sites and its own
+ * exact fallback.  gate_errors / used_exact (host, batch entries each, may be
+ * NULL) as for lmg_surrogate_map_dev.  lmg_surrogate_map_dev is this call with
+ * batch 1. */
+lmg_status lmg_surrogate_map_batch_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q,
+                                       const int32_t* counts, int32_t batch, float* d_map, double tolerance,
+                                       int32_t gate_sites, uint64_t gate_seed, double* gate_errors,
+                                       int32_t* used_exact);
+/* Outcomes of the latest gated call's first `batch` maps (waits for the stream). */
+lmg_status lmg_surrogate_last_gates(lmg_surrogate* s, int32_t batch, double* gate_errors, int32_t* used_exact);
 /* lmg_surrogate_map_dev records its launches once as a HIP graph and replays
  * it while the pointers, sizes and tolerance stay the same (default on); off
  * launches every kernel each call. */

@@ -140,6 +140,8 @@ def lib():
     L.lmg_surrogate_train_dev.argtypes = [VP, VP, VP, I32, I32, I32, F64, U64, VP]
     L.lmg_surrogate_map_dev.argtypes = [VP, VP, VP, I32, VP, F64, I32, U64, PF64, PI32]
     L.lmg_surrogate_last_gate.argtypes = [VP, PF64, PI32]
+    L.lmg_surrogate_map_batch_dev.argtypes = [VP, VP, VP, PI32, I32, VP, F64, I32, U64, PF64, PI32]
+    L.lmg_surrogate_last_gates.argtypes = [VP, I32, PF64, PI32]
     L.lmg_surrogate_set_graphs.argtypes = [VP, I32]
     L.lmg_potential_map_if_dev.argtypes = [VP, VP, VP, I32, I32, VP, VP]
     _lib = L
@@ -161,4 +163,5 @@ EXPORTED = (
     "lmg_surrogate_get_weights", "lmg_surrogate_set_weights", "lmg_surrogate_occupancy_dev",
     "lmg_surrogate_infer_dev", "lmg_surrogate_gradient_dev", "lmg_surrogate_train_dev",
     "lmg_surrogate_map_dev", "lmg_surrogate_last_gate", "lmg_surrogate_set_graphs", "lmg_potential_map_if_dev",
+    "lmg_surrogate_map_batch_dev", "lmg_surrogate_last_gates",
 )

@@ -61,11 +61,8 @@ __global__ void to_float4_kernel(Dims d, const int* __restrict__ site, const int
 // charges [s*chunk, (s+1)*chunk).  part[s*ld + t] = (sum, compensation).
 // ===========================================================================
 template <bool kGauss, bool kImage>
-__global__ __launch_bounds__(kBlock) void pair_kernel(const float4* __restrict__ tgt, int nt,
-                                                      const float4* __restrict__ chg, int nc,
-                                                      uint32_t c2m1, float gk,
-                                                      float2* __restrict__ part, int ld) {
-    __shared__ float4 tile[kTile];
+__device__ __forceinline__ void pair_body(const float4* __restrict__ tgt, int nt, const float4* __restrict__ chg, int nc,
+                                          uint32_t c2m1, float gk, float2* __restrict__ part, int ld, float4* tile) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int S = gridDim.y;
     const int chunk = (nc + S - 1) / S;
@@ -96,6 +93,28 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(const float4* __restrict__
     if (t < nt) part[blockIdx.y * ld + t] = make_float2(s, -c);
 }
 
+template <bool kGauss, bool kImage>
+__global__ __launch_bounds__(kBlock) void pair_kernel(const float4* __restrict__ tgt, int nt,
+                                                      const float4* __restrict__ chg, int nc,
+                                                      uint32_t c2m1, float gk,
+                                                      float2* __restrict__ part, int ld) {
+    __shared__ float4 tile[kTile];
+    pair_body<kGauss, kImage>(tgt, nt, chg, nc, c2m1, gk, part, ld, tile);
+}
+
+// The same targets for a batch of charge sets: set b = blockIdx.z holds
+// charges [off[b], off[b+1]) and writes its partials at part + b * S * ld.
+template <bool kGauss>
+__global__ __launch_bounds__(kBlock) void pair_batch_kernel(const float4* __restrict__ tgt, int nt,
+                                                            const float4* __restrict__ chg,
+                                                            const int* __restrict__ off, uint32_t c2m1, float gk,
+                                                            float2* __restrict__ part, int ld) {
+    __shared__ float4 tile[kTile];
+    const int b = blockIdx.z, o = off[b];
+    pair_body<kGauss, false>(tgt, nt, chg + o, off[b + 1] - o, c2m1, gk,
+                             part + (long long)b * gridDim.y * ld, ld, tile);
+}
+
 // out[t] = scale * sum_s part[s][t]: 16 lanes per target, lane j summing chunks
 // s = j, j + 16, ... in order, then the 16 lane sums in lane order (FP64).
 constexpr int kRedLanes = 16;
@@ -124,16 +143,12 @@ __global__ void pair_reduce_kernel(const float2* __restrict__ part, int nt, int 
 // row; a workgroup owns (tx * kMapT) x (256 / tx) sites of one z plane.
 // ===========================================================================
 template <bool kGauss>
-__global__ __launch_bounds__(kBlock) void map_kernel(Dims d, const float4* __restrict__ chg, int nc,
-                                                     uint32_t c2m1, float gk, float pref, int tx_n,
-                                                     const int32_t* __restrict__ run, float* __restrict__ out) {
-    if (run && *run == 0) return;   // conditional map whose device-side condition is false
-    __shared__ float4 tile[kTile];
+__device__ __forceinline__ void map_body(Dims d, const float4* __restrict__ chg, int nc, uint32_t c2m1, float gk,
+                                         float pref, int tx_n, int z, float* __restrict__ out, float4* tile) {
     const int tx = threadIdx.x % tx_n, ty = threadIdx.x / tx_n;
     const int ty_n = kBlock / tx_n;
     const int x0 = (blockIdx.x * tx_n + tx) * kMapT;
     const int y = blockIdx.y * ty_n + ty;
-    const int z = blockIdx.z;
     const float fy = float(y), fz = float(z);
     float s[kMapT], c[kMapT];
 #pragma unroll
@@ -174,6 +189,30 @@ __global__ __launch_bounds__(kBlock) void map_kernel(Dims d, const float4* __res
             if (x < d.X) out[encode(d, x, y, z)] = pref * (s[u] - c[u]);
         }
     }
+}
+
+template <bool kGauss>
+__global__ __launch_bounds__(kBlock) void map_kernel(Dims d, const float4* __restrict__ chg, int nc,
+                                                     uint32_t c2m1, float gk, float pref, int tx_n,
+                                                     const int32_t* __restrict__ run, float* __restrict__ out) {
+    if (run && *run == 0) return;   // conditional map whose device-side condition is false
+    __shared__ float4 tile[kTile];
+    map_body<kGauss>(d, chg, nc, c2m1, gk, pref, tx_n, blockIdx.z, out, tile);
+}
+
+// A batch of conditional maps: grid z = Z * batch; map b = blockIdx.z / Z of
+// charges [off[b], off[b+1]) into out + b * V when run[b] is set.
+template <bool kGauss>
+__global__ __launch_bounds__(kBlock) void map_batch_kernel(Dims d, const float4* __restrict__ chg,
+                                                           const int* __restrict__ off, uint32_t c2m1, float gk,
+                                                           float pref, int tx_n, const int32_t* __restrict__ run,
+                                                           float* __restrict__ out) {
+    const int b = blockIdx.z / d.Z;
+    if (run[b] == 0) return;
+    __shared__ float4 tile[kTile];
+    const int o = off[b];
+    map_body<kGauss>(d, chg + o, off[b + 1] - o, c2m1, gk, pref, tx_n, blockIdx.z - b * d.Z,
+                     out + (long long)b * d.V, tile);
 }
 
 // ---------------------------------------------------------------------------

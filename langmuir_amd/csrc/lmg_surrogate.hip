@@ -47,7 +47,7 @@ constexpr double kStepFactor = 0.9;
 constexpr int kPrecondSnapshots = 256;   // occupancies averaged into the preconditioner
 constexpr double kPrecondFloor = 1e-3;   // relative floor of the spectral power
 constexpr int kGateMaxSplit = 128;       // charge chunks of the gate's pair sums (split_count's cap)
-constexpr int kMaxGateSites = kRedBlocks * 16;   // gate_reduce_kernel's block sums fit the partial buffer
+constexpr int kMaxGateSites = 8192;      // gate sites per map
 
 // Spectrum layout (one sample): complex values (re, im) at index
 // k = (kx Py + ky) Pz + kz, i.e. rows (kx, ky) of Pz interleaved pairs.
@@ -301,19 +301,64 @@ __global__ void occ_scatter_f4_kernel(const int32_t* __restrict__ site, const in
     } while (old != assumed);
 }
 
+// occ_scatter_f4_kernel for a batch of charge lists: charge i belongs to list b
+// with off[b] <= i < off[b+1] and goes into grid occ + b * V.
+__global__ void occ_scatter_batch_kernel(const int32_t* __restrict__ site, const int32_t* __restrict__ q,
+                                         const int* __restrict__ off, int batch, lmg::Dims d, int8_t* occ, int* bad,
+                                         float4* __restrict__ chg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= off[batch]) return;
+    int lo = 0, hi = batch;   // the last b with off[b] <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    const int s = site[i];
+    const int qi = q[i];
+    if (s < 0 || s >= d.V || qi < -127 || qi > 127) {
+        atomicOr(bad, 1);
+        chg[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    int x, y, z;
+    lmg::decode(d, s, x, y, z);
+    chg[i] = make_float4(float(x), float(y), float(z), float(qi));
+    int8_t* g = occ + (long long)lo * d.V;
+    unsigned int* w = reinterpret_cast<unsigned int*>(g + (s & ~3));
+    const int sh = (s & 3) * 8;
+    unsigned int old = *w, assumed;
+    do {
+        assumed = old;
+        const int b = int(int8_t((assumed >> sh) & 0xffu)) + qi;
+        const unsigned int nv = (assumed & ~(0xffu << sh)) | (unsigned(uint8_t(int8_t(b))) << sh);
+        old = atomicCAS(w, assumed, nv);
+    } while (old != assumed);
+}
+
 // The gate in one launch: the exact potential at each gate site (the S chunk
 // partials of pair_kernel summed in FP64 like pair_reduce_kernel, 16 lanes per
 // site), its squared difference from the map, block sums, and in the last block
 // to finish the total over blocks in block order: err = sqrt(sum d^2 / sum e^2)
 // and *run = err > tol.  The finishing counter returns to zero for the next call.
+// Map b = blockIdx.y of a batch uses part + b S nt, map + b V, bpart + 2 b
+// gridDim.x and entry b of counter, err and run.
 constexpr int kGateLanes = 16;
 __global__ __launch_bounds__(256) void gate_reduce_kernel(const float2* __restrict__ part, int nt, int S, double scale,
-                                                          const float* __restrict__ map,
+                                                          const float* __restrict__ map, long long V,
                                                           const int32_t* __restrict__ site, double* __restrict__ bpart,
                                                           unsigned int* __restrict__ counter, double tol,
                                                           double* __restrict__ err, int32_t* __restrict__ run) {
     __shared__ double sh[256];
     __shared__ bool last;
+    {
+        const int b = blockIdx.y;
+        part += (long long)b * S * nt;
+        map += (long long)b * V;
+        bpart += 2ll * b * gridDim.x;
+        counter += b;
+        err += b;
+        run += b;
+    }
     const int j = threadIdx.x % kGateLanes;
     const int t = blockIdx.x * (256 / kGateLanes) + threadIdx.x / kGateLanes;
     double v = 0.0;
@@ -391,26 +436,32 @@ struct lmg_surrogate {
     lmg::Dims dims{};
     int32_t* gate_site = nullptr;  // sampled sites
     float4* gate_tgt = nullptr;    // their coordinates
-    float2* gate_part = nullptr;   // pair-sum partials, kGateMaxSplit x gate_cap
+    float2* gate_part = nullptr;   // pair-sum partials: S x gate sites per map of the batch
+    long long gate_part_cap = 0;   // float2 entries
+    double* gate_bpart = nullptr;  // gate_reduce_kernel's block sums, 2 per block per map
+    long long gate_bpart_cap = 0;
     int gate_cap = 0;
     int gate_n = 0;                // sites currently in gate_site, and their seed
     uint64_t gate_seed = 0;
-    float4* chg = nullptr;         // charges as (x, y, z, q)
-    int chg_cap = 0;
-    unsigned int* gate_counter = nullptr;   // blocks finished in gate_reduce_kernel (0 between calls)
-    double* gate_err = nullptr;    // device: latest gate error
-    int32_t* gate_run = nullptr;   // device: latest decision (1 = exact map used)
+    float4* chg = nullptr;         // charges as (x, y, z, q), the batch's lists one after another
+    long long chg_cap = 0;
+    int* off = nullptr;            // device: list offsets of the batch (max_batch + 1)
+    std::vector<int> off_host;     // what off holds
+    unsigned int* gate_counter = nullptr;   // per map: blocks finished in gate_reduce_kernel (0 between calls)
+    double* gate_err = nullptr;    // device, per map: latest gate error
+    int32_t* gate_run = nullptr;   // device, per map: latest decision (1 = exact map used)
     // the gated map as a replayable graph, valid while its key holds
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
     unsigned buf_gen = 0;          // bumped whenever a gate/charge buffer is reallocated
     struct Key {
         const void *site, *q, *map;
-        int n, gate_sites;
+        std::vector<int> off;
+        int gate_sites;
         double tol;
         unsigned gen;
         bool operator==(const Key& o) const {
-            return site == o.site && q == o.q && map == o.map && n == o.n && gate_sites == o.gate_sites &&
+            return site == o.site && q == o.q && map == o.map && off == o.off && gate_sites == o.gate_sites &&
                    tol == o.tol && gen == o.gen;
         }
     } gkey{};
@@ -737,15 +788,16 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
         (st = salloc(s, (void**)&s->tgt, size_t(s->V) * max_batch * 4)) ||
         (st = salloc(s, (void**)&s->part, kRedBlocks * 2 * sizeof(double))) ||
         (st = salloc(s, (void**)&s->red, 8 * sizeof(double))) || (st = salloc(s, (void**)&s->bad, sizeof(int))) ||
-        (st = salloc(s, (void**)&s->gate_err, sizeof(double))) ||
-        (st = salloc(s, (void**)&s->gate_counter, sizeof(unsigned int))) ||
-        (st = salloc(s, (void**)&s->gate_run, sizeof(int32_t))))
+        (st = salloc(s, (void**)&s->gate_err, size_t(max_batch) * sizeof(double))) ||
+        (st = salloc(s, (void**)&s->gate_counter, size_t(max_batch) * sizeof(unsigned int))) ||
+        (st = salloc(s, (void**)&s->gate_run, size_t(max_batch) * sizeof(int32_t))) ||
+        (st = salloc(s, (void**)&s->off, size_t(max_batch + 1) * sizeof(int))))
         return fail_create(st);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->w), s->nw * 2, 0u);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->mom), s->nw * 2, 0u);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s->stream, s->gate_counter, 1LL, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s->stream, s->gate_counter, (long long)max_batch, 0u);
     if ((st = launch_ok(s, "surrogate init")) != LMG_OK) return fail_create(st);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return fail_create(sfail(s, LMG_ERR_HIP, "init sync"));
     *out = s;
@@ -961,49 +1013,59 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
 
 namespace {
 
-// Every launch of one gated map, on s->stream (also what a graph captures):
-// occupancy and charge coordinates, the forward pass, the exact pair sums at the
-// gate sites, the gate, and the exact map when the gate failed.
-void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int n, float* d_map,
-                   double tolerance, int gate_sites) {
+// Every launch of a batch of gated maps, on s->stream (also what a graph
+// captures): occupancy grids and charge coordinates, the forward pass, the
+// exact pair sums at the gate sites for every map, the gates, and the exact
+// map of every map whose gate failed.  Map b's charges are entries
+// [off[b], off[b+1]) of the lists; S is the gate's charge split.
+void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int batch, long long total,
+                   float* d_map, double tolerance, int gate_sites, int S) {
     const hipStream_t st = s->stream;
-    const long long words = (s->V + 3) / 4;
+    const long long words = (s->V * batch + 3) / 4;
     hipLaunchKernelGGL(occ_clear_kernel, dim3(grid_for(words)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(s->occ),
                        words, s->bad);
-    if (n > 0)
-        hipLaunchKernelGGL(occ_scatter_f4_kernel, dim3((n + 255) / 256), dim3(256), 0, st, d_site, d_q, n, s->dims,
-                           s->occ, s->bad, s->chg);
-    forward(s, s->w, s->occ, 1, d_map);
+    if (total > 0)
+        hipLaunchKernelGGL(occ_scatter_batch_kernel, dim3(int((total + 255) / 256)), dim3(256), 0, st, d_site, d_q,
+                           s->off, batch, s->dims, s->occ, s->bad, s->chg);
+    forward(s, s->w, s->occ, batch, d_map);
     const bool gauss = s->flags & LMG_GAUSS;
     const uint32_t c2m1 = lmg::cutoff_bits(s->p.cutoff, s->flags & LMG_NO_CUTOFF);
     const float gk = lmg::gauss_k(s->p.gaussian_sigma);
-    const int S = lmg::split_count(gate_sites, n);
-    const dim3 pgrid((gate_sites + lmg::kBlock - 1) / lmg::kBlock, S);
+    const dim3 pgrid((gate_sites + lmg::kBlock - 1) / lmg::kBlock, S, batch);
     if (gauss)
-        hipLaunchKernelGGL((lmg::pair_kernel<true, false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
-                           s->chg, n, c2m1, gk, s->gate_part, gate_sites);
+        hipLaunchKernelGGL((lmg::pair_batch_kernel<true>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
+                           s->chg, s->off, c2m1, gk, s->gate_part, gate_sites);
     else
-        hipLaunchKernelGGL((lmg::pair_kernel<false, false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
-                           s->chg, n, c2m1, gk, s->gate_part, gate_sites);
+        hipLaunchKernelGGL((lmg::pair_batch_kernel<false>), pgrid, dim3(lmg::kBlock), 0, st, s->gate_tgt, gate_sites,
+                           s->chg, s->off, c2m1, gk, s->gate_part, gate_sites);
     const int per = 256 / kGateLanes;
-    hipLaunchKernelGGL(gate_reduce_kernel, dim3((gate_sites + per - 1) / per), dim3(256), 0, st, s->gate_part,
-                       gate_sites, S, s->p.prefactor, d_map, s->gate_site, s->part, s->gate_counter, tolerance,
-                       s->gate_err, s->gate_run);
-    // exact map over the surrogate's when the gate failed (decided on the device)
+    hipLaunchKernelGGL(gate_reduce_kernel, dim3((gate_sites + per - 1) / per, batch), dim3(256), 0, st, s->gate_part,
+                       gate_sites, S, s->p.prefactor, d_map, s->V, s->gate_site, s->gate_bpart, s->gate_counter,
+                       tolerance, s->gate_err, s->gate_run);
+    // exact maps over the surrogate's where the gate failed (decided on the device)
     int tx = 1;
     while (tx < 64 && tx * lmg::kMapT < s->nx) tx <<= 1;
     const int ty = lmg::kBlock / tx;
-    const dim3 mgrid((s->nx + tx * lmg::kMapT - 1) / (tx * lmg::kMapT), (s->ny + ty - 1) / ty, s->nz);
+    const dim3 mgrid((s->nx + tx * lmg::kMapT - 1) / (tx * lmg::kMapT), (s->ny + ty - 1) / ty, s->nz * batch);
     const float pref = float(s->p.prefactor);
     if (gauss)
-        hipLaunchKernelGGL(lmg::map_kernel<true>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, n, c2m1, gk, pref,
-                           tx, s->gate_run, d_map);
+        hipLaunchKernelGGL(lmg::map_batch_kernel<true>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, s->off, c2m1,
+                           gk, pref, tx, s->gate_run, d_map);
     else
-        hipLaunchKernelGGL(lmg::map_kernel<false>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, n, c2m1, gk,
-                           pref, tx, s->gate_run, d_map);
+        hipLaunchKernelGGL(lmg::map_batch_kernel<false>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, s->off,
+                           c2m1, gk, pref, tx, s->gate_run, d_map);
 }
 
-lmg_status gate_buffers(lmg_surrogate* s, int n, int gate_sites, uint64_t gate_seed) {
+// The gate's charge split: about 4096 waves over the batch, >= 64 charges per
+// chunk of the longest list, at most kGateMaxSplit.
+int gate_split(int gate_sites, int max_n, int batch) {
+    const long long waves = (long long)((gate_sites + 63) / 64) * batch;
+    long long S = (4096 + waves - 1) / waves;
+    S = std::min<long long>(S, std::max(1, max_n / 64));
+    return int(std::max<long long>(1, std::min<long long>(S, kGateMaxSplit)));
+}
+
+lmg_status gate_buffers(lmg_surrogate* s, long long total, int gate_sites, uint64_t gate_seed, int batch, int S) {
     lmg_status st;
     auto regrow = [&](void** p, size_t bytes) -> lmg_status {
         if (*p) {
@@ -1017,14 +1079,23 @@ lmg_status gate_buffers(lmg_surrogate* s, int n, int gate_sites, uint64_t gate_s
     };
     if (gate_sites > s->gate_cap) {
         if ((st = regrow((void**)&s->gate_site, size_t(gate_sites) * 4)) ||
-            (st = regrow((void**)&s->gate_tgt, size_t(gate_sites) * sizeof(float4))) ||
-            (st = regrow((void**)&s->gate_part, size_t(kGateMaxSplit) * gate_sites * sizeof(float2))))
+            (st = regrow((void**)&s->gate_tgt, size_t(gate_sites) * sizeof(float4))))
             return st;
         s->gate_cap = gate_sites;
         s->gate_n = 0;
     }
-    if (n > s->chg_cap) {
-        const int cap = std::max(n, 2 * s->chg_cap);
+    const long long parts = (long long)S * gate_sites * batch;
+    if (parts > s->gate_part_cap) {
+        if ((st = regrow((void**)&s->gate_part, size_t(parts) * sizeof(float2))) != LMG_OK) return st;
+        s->gate_part_cap = parts;
+    }
+    const long long bparts = 2ll * ((gate_sites + 256 / kGateLanes - 1) / (256 / kGateLanes)) * batch;
+    if (bparts > s->gate_bpart_cap) {
+        if ((st = regrow((void**)&s->gate_bpart, size_t(bparts) * sizeof(double))) != LMG_OK) return st;
+        s->gate_bpart_cap = bparts;
+    }
+    if (total > s->chg_cap) {
+        const long long cap = std::max(total, 2 * s->chg_cap);
         if ((st = regrow((void**)&s->chg, size_t(cap) * sizeof(float4))) != LMG_OK) return st;
         s->chg_cap = cap;
     }
@@ -1039,24 +1110,60 @@ lmg_status gate_buffers(lmg_surrogate* s, int n, int gate_sites, uint64_t gate_s
     return launch_ok(s, "gate set-up");
 }
 
-}  // namespace
-
-lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n, float* d_map,
-                                 double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
-                                 int32_t* used_exact) {
-    if (!s) return LMG_ERR_ARG;
-    if (n < 0 || !d_map || (n > 0 && (!d_site || !d_q)) || !(tolerance >= 0))
-        return sfail(s, LMG_ERR_ARG, "invalid map arguments");
-    if (gate_sites < 1 || gate_sites > kMaxGateSites)
-        return sfail(s, LMG_ERR_ARG, "gate_sites must be in [1, " + std::to_string(kMaxGateSites) + "]");
+lmg_status read_gates(lmg_surrogate* s, int batch, double* gate_error, int32_t* used_exact) {
     lmg_status st;
     if ((st = use_dev(s)) != LMG_OK) return st;
-    if ((st = gate_buffers(s, n, gate_sites, gate_seed)) != LMG_OK) return st;
+    std::vector<double> e(batch);
+    std::vector<int32_t> run(batch);
+    int32_t bad = 0;
+    SHIP(s, hipMemcpyAsync(e.data(), s->gate_err, batch * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipMemcpyAsync(run.data(), s->gate_run, batch * sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipMemcpyAsync(&bad, s->bad, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipStreamSynchronize(s->stream));
+    if (bad) return sfail(s, LMG_ERR_ARG, "charge site outside the lattice or |q| > 127");
+    if (gate_error) std::copy(e.begin(), e.end(), gate_error);
+    if (used_exact) std::copy(run.begin(), run.end(), used_exact);
+    return LMG_OK;
+}
+
+}  // namespace
+
+lmg_status lmg_surrogate_map_batch_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q,
+                                       const int32_t* counts, int32_t batch, float* d_map, double tolerance,
+                                       int32_t gate_sites, uint64_t gate_seed, double* gate_errors,
+                                       int32_t* used_exact) {
+    if (!s) return LMG_ERR_ARG;
+    if (batch < 1 || batch > s->cap || !counts || !d_map || !(tolerance >= 0))
+        return sfail(s, LMG_ERR_ARG, "invalid map arguments (batch must be in [1, max_batch])");
+    if ((long long)s->nz * batch > 65535) return sfail(s, LMG_ERR_CAPACITY, "grid.z x batch above 65535");
+    if (gate_sites < 1 || gate_sites > kMaxGateSites)
+        return sfail(s, LMG_ERR_ARG, "gate_sites must be in [1, " + std::to_string(kMaxGateSites) + "]");
+    std::vector<int> off(batch + 1, 0);
+    long long total = 0;
+    int max_n = 0;
+    for (int b = 0; b < batch; ++b) {
+        if (counts[b] < 0) return sfail(s, LMG_ERR_ARG, "negative charge count");
+        total += counts[b];
+        if (total > INT32_MAX) return sfail(s, LMG_ERR_CAPACITY, "more than 2^31 - 1 charges in a batch");
+        off[b + 1] = int(total);
+        max_n = std::max(max_n, int(counts[b]));
+    }
+    if (total > 0 && (!d_site || !d_q)) return sfail(s, LMG_ERR_ARG, "invalid map arguments");
+    const int S = gate_split(gate_sites, max_n, batch);
+    lmg_status st;
+    if ((st = use_dev(s)) != LMG_OK) return st;
+    if ((st = gate_buffers(s, total, gate_sites, gate_seed, batch, S)) != LMG_OK) return st;
+    if (off != s->off_host) {   // the offsets are read by the kernels (and by replays) from device memory
+        SHIP(s, hipStreamSynchronize(s->stream));
+        s->off_host = off;
+        SHIP(s, hipMemcpyAsync(s->off, s->off_host.data(), (batch + 1) * sizeof(int), hipMemcpyHostToDevice,
+                               s->stream));
+    }
     if (!s->use_graphs) {
-        enqueue_gated(s, d_site, d_q, n, d_map, tolerance, gate_sites);
+        enqueue_gated(s, d_site, d_q, batch, total, d_map, tolerance, gate_sites, S);
         if ((st = launch_ok(s, "surrogate map")) != LMG_OK) return st;
     } else {
-        const lmg_surrogate::Key key{d_site, d_q, d_map, n, gate_sites, tolerance, s->buf_gen};
+        lmg_surrogate::Key key{d_site, d_q, d_map, off, gate_sites, tolerance, s->buf_gen};
         if (!s->gexec || !(key == s->gkey)) {
             if (s->gexec) {
                 SHIP(s, hipStreamSynchronize(s->stream));
@@ -1065,7 +1172,7 @@ lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const 
             }
             hipGraph_t graph = nullptr;
             SHIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
-            enqueue_gated(s, d_site, d_q, n, d_map, tolerance, gate_sites);
+            enqueue_gated(s, d_site, d_q, batch, total, d_map, tolerance, gate_sites, S);
             const hipError_t cap = hipStreamEndCapture(s->stream, &graph);
             if (cap != hipSuccess) return sfail(s, LMG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(cap));
             const hipError_t inst = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
@@ -1078,24 +1185,28 @@ lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const 
         }
         SHIP(s, hipGraphLaunch(s->gexec, s->stream));
     }
-    if (!gate_error && !used_exact) return LMG_OK;
-    return lmg_surrogate_last_gate(s, gate_error, used_exact);
+    if (!gate_errors && !used_exact) return LMG_OK;
+    return read_gates(s, batch, gate_errors, used_exact);
+}
+
+lmg_status lmg_surrogate_map_dev(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, int32_t n, float* d_map,
+                                 double tolerance, int32_t gate_sites, uint64_t gate_seed, double* gate_error,
+                                 int32_t* used_exact) {
+    if (!s) return LMG_ERR_ARG;
+    if (n < 0) return sfail(s, LMG_ERR_ARG, "invalid map arguments");
+    return lmg_surrogate_map_batch_dev(s, d_site, d_q, &n, 1, d_map, tolerance, gate_sites, gate_seed, gate_error,
+                                       used_exact);
 }
 
 lmg_status lmg_surrogate_last_gate(lmg_surrogate* s, double* gate_error, int32_t* used_exact) {
     if (!s) return LMG_ERR_ARG;
-    lmg_status st;
-    if ((st = use_dev(s)) != LMG_OK) return st;
-    double e = 0.0;
-    int32_t run = 0, bad = 0;
-    SHIP(s, hipMemcpyAsync(&e, s->gate_err, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    SHIP(s, hipMemcpyAsync(&run, s->gate_run, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
-    SHIP(s, hipMemcpyAsync(&bad, s->bad, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
-    SHIP(s, hipStreamSynchronize(s->stream));
-    if (bad) return sfail(s, LMG_ERR_ARG, "charge site outside the lattice or |q| > 127");
-    if (gate_error) *gate_error = e;
-    if (used_exact) *used_exact = run;
-    return LMG_OK;
+    return read_gates(s, 1, gate_error, used_exact);
+}
+
+lmg_status lmg_surrogate_last_gates(lmg_surrogate* s, int32_t batch, double* gate_errors, int32_t* used_exact) {
+    if (!s) return LMG_ERR_ARG;
+    if (batch < 1 || batch > s->cap) return sfail(s, LMG_ERR_ARG, "batch must be in [1, max_batch]");
+    return read_gates(s, batch, gate_errors, used_exact);
 }
 
 }  // extern "C"

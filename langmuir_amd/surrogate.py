@@ -123,6 +123,34 @@ class Surrogate:
                                                    C.byref(used)))
         return float(err.value), bool(used.value)
 
+    def map_batch_dev(self, site_ptr: int, q_ptr: int, counts: Sequence[int], map_ptr: int,
+                      tolerance: float = 1e-3, gate_sites: int = 1024, gate_seed: int = 1, wait: bool = True):
+        """A batch of gated maps: map b of the charges counts[0] + ... + counts[b-1]
+        onwards (counts[b] of them) into map_ptr + 4 b V.  Returns (errors, used
+        exact) per map, or None with wait=False (see :meth:`last_gates`)."""
+        c = np.ascontiguousarray(np.asarray(counts, np.int32))
+        b = len(c)
+        pc = c.ctypes.data_as(_abi.PI32)
+        if not wait:
+            self._chk(_abi.lib().lmg_surrogate_map_batch_dev(self._h, site_ptr, q_ptr, pc, b, map_ptr,
+                                                             float(tolerance), int(gate_sites),
+                                                             C.c_uint64(gate_seed), None, None))
+            return None
+        err = np.zeros(b, np.float64)
+        used = np.zeros(b, np.int32)
+        self._chk(_abi.lib().lmg_surrogate_map_batch_dev(self._h, site_ptr, q_ptr, pc, b, map_ptr, float(tolerance),
+                                                         int(gate_sites), C.c_uint64(gate_seed),
+                                                         err.ctypes.data_as(_abi.PF64),
+                                                         used.ctypes.data_as(_abi.PI32)))
+        return err, used.astype(bool)
+
+    def last_gates(self, batch: int):
+        err = np.zeros(int(batch), np.float64)
+        used = np.zeros(int(batch), np.int32)
+        self._chk(_abi.lib().lmg_surrogate_last_gates(self._h, int(batch), err.ctypes.data_as(_abi.PF64),
+                                                      used.ctypes.data_as(_abi.PI32)))
+        return err, used.astype(bool)
+
     def set_graphs(self, on: bool) -> None:
         """Replay the gated map as a recorded HIP graph (default) or launch
         every kernel each call."""

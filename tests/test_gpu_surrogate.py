@@ -140,6 +140,42 @@ def test_graph_replays_match_eager_launches(gpu):
         assert np.array_equal(d_out.download(np.float32, V), eager)
 
 
+def test_batched_gated_maps_match_single_calls(gpu, oracle):
+    """lmg_surrogate_map_batch_dev: every map of a batch (lists of different
+    lengths, one empty) equals the single gated call on its list -- the map bit
+    for bit, the gate decision, and the gate error to within 1e-6 (its exact pair
+    sums are split over charges differently in a batch, so they differ in FP32
+    rounding) -- for accepted (ideal weights) and rejected (untrained) gates."""
+    dims, cutoff = (28, 24, 8), 7
+    rng = np.random.default_rng(11)
+    V = int(np.prod(dims))
+    ctx, sur = make(gpu, dims, cutoff, max_batch=4)
+    lists = [random_charges(rng, V, 150), random_charges(rng, V, 40), (np.zeros(0, np.int32), np.zeros(0, np.int32)),
+             random_charges(rng, V, 90)]
+    counts = [len(a) for a, _ in lists]
+    d_s = gpu.DeviceBuffer.from_array(np.concatenate([a for a, _ in lists]))
+    d_q = gpu.DeviceBuffer.from_array(np.concatenate([b for _, b in lists]))
+    d_out = gpu.DeviceBuffer(4 * V * len(lists))
+    for weights in (S.ideal_weights(dims, cutoff, PREF), None):
+        sur.set_weights(weights)
+        d_out.upload(np.full(V * len(lists), np.nan, np.float32))
+        errs, used = sur.map_batch_dev(d_s.ptr, d_q.ptr, counts, d_out.ptr, 1e-3, gate_sites=512)
+        got = d_out.download(np.float32, V * len(lists)).reshape(len(lists), V)
+        for b, (sites, q) in enumerate(lists):
+            out, err, u = sur.map(sites, q, tolerance=1e-3, gate_sites=512)
+            assert u == used[b], b
+            assert np.array_equal(got[b], out), b
+            # the gate's exact sums differ in FP32 rounding (about 1e-7 relative per site)
+            assert abs(err - errs[b]) <= 1e-6 + 1e-6 * err, (b, err, errs[b])
+            if len(sites) and weights is not None:
+                assert rel_rms(got[b], oracle.potential_map(dims, cutoff, sites, q, pref=PREF)) <= 1e-4
+        expect = [False] * 4 if weights is not None else [True, True, False, True]   # an empty list's map is 0
+        assert used.tolist() == expect
+    sur.map_batch_dev(d_s.ptr, d_q.ptr, counts, d_out.ptr, 1e-3, gate_sites=512, wait=False)
+    e2, u2 = sur.last_gates(len(lists))
+    assert np.array_equal(e2, errs) and np.array_equal(u2, used)
+
+
 def solar_snapshots(gpu, sur, dims, cutoff, count, stride, seed):
     p = gpu.SimulationParameters(
         simulation_type="solarcell", grid_x=dims[0], grid_y=dims[1], grid_z=dims[2], coulomb_carriers=True,
