@@ -91,3 +91,46 @@ def test_checkpoint_resume_is_exact(gpu, tmp_path):
     _, rows_a = L.read_flux_table(str(tmp_path / "a" / "run.dat"))
     _, rows_c = L.read_flux_table(str(tmp_path / "c" / "run.dat"))
     assert np.array_equal(rows_a[2:, :23], rows_c[:, :23])
+
+
+def oracle_params(oracle, sp):
+    """OrcParams with the values of a SimulationParameters."""
+    kw = {}
+    for name, _ in oracle.OrcParams._fields_:
+        if name == "reserved0":
+            continue
+        v = getattr(sp, name)
+        if name == "simulation_type":
+            v = 1 if sp.is_solarcell else 0
+        elif isinstance(v, bool):
+            v = int(v)
+        kw[name] = v
+    return oracle.OrcParams.make(**kw)
+
+
+@pytest.mark.parametrize("name,seed", [("transistor.inp", 11), ("solar.inp", 12), ("sample.inp", 4242)])
+def test_reference_example_inputs_step_like_the_oracle(gpu, oracle, name, seed):
+    """The reference's own example inputs (examples/*.inp), read and set up by
+    langmuir_amd.io / world, step on the device exactly as the oracle steps the same
+    world (near-threshold Metropolis ties aside, as in test_gpu_sim).  The files
+    leave random.seed at 0 (a clock seed); the test fixes one instead."""
+    from _util import resync   # noqa: F401  (run_parity resyncs through it)
+    from test_gpu_sim import run_parity
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_files", name)
+    ck = L.read_checkpoint(path)
+    ck.parameters["random.seed"] = seed
+    setup = L.setup_of(ck)
+    sp = setup.params
+    print(f"{name}: {sp.grid_x}x{sp.grid_y}x{sp.grid_z} {sp.simulation_type}, {len(setup.electrons)} e, "
+          f"{len(setup.holes)} h, {len(setup.traps)} traps", flush=True)
+    w = oracle.World(oracle_params(oracle, sp), electrons=setup.electrons, holes=setup.holes,
+                     defects=setup.defects, traps=setup.traps, trap_potentials=setup.trap_potentials)
+    w.set_rng(setup.rng_window, setup.rng_consumed)
+    assert np.array_equal(w.potential(), setup.potential)
+    sim = gpu.Simulation(sp, **L.simulation_kwargs(ck, setup))
+    sim.set_synchronous(True)
+    mism = run_parity(w, sim, 40, sp.inverse_kt)
+    assert mism <= 2
+    assert sim.device_error() == 0
+    if name == "transistor.inp":   # the left electron source injects at 0.9 per step
+        assert len(sim.state()["e_site"]) > 0
