@@ -23,8 +23,9 @@ import re
 import io as _io
 import os
 import time
+from collections import deque
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Deque, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -319,8 +320,6 @@ def _open_text(path: str) -> str:
     return raw.decode("latin-1")
 
 
-
-
 def fix_traps(ck: Checkpoint) -> None:
     """LangmuirPython's CheckPoint.fix_traps (LangmuirPython/langmuir/checkpoint.py:337-388):
     trap.percentage = (n + 0.25) / V, so that int(trap.percentage * V) is the listed n and
@@ -361,7 +360,7 @@ def parse_checkpoint(text: str) -> Checkpoint:
     after it is not read (checkpointer.cpp:403-428)."""
     ck = Checkpoint()
     stream = _io.StringIO(text)
-    buf: List[str] = []
+    buf: Deque[str] = deque()
 
     def next_token() -> Optional[str]:
         while not buf:
@@ -369,7 +368,7 @@ def parse_checkpoint(text: str) -> Checkpoint:
             if not line:
                 return None
             buf.extend(line.split())
-        return buf.pop(0)
+        return buf.popleft()
 
     def uint(tok: str) -> int:
         if not _INT_TOKEN.match(tok) or not 0 <= int(tok) < (1 << 32):
