@@ -12,8 +12,7 @@
 // and lo BF16 planes) transposed, as Bc^T[n][k], rows padded to a multiple of
 // 64 and k to a multiple of 32 with zeros; each wave reads its fragments of it
 // straight from global memory (they stay in L2).  The data tiles go through
-// LDS, k contiguous, double-buffered, with a whole chunk of them loaded into
-// registers at once.
+// LDS, k contiguous, double-buffered.
 //
 // Output: row-major C[row * ldc + col], or transposed for the next stage of a
 // separable transform: C at
@@ -32,7 +31,6 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kTM = 64, kTN = 64, kTK = 32;   // block tile
-constexpr int kChunk = 10;                    // K tiles whose data loads are in flight together
 constexpr int kLds = kTK + 8;                 // LDS row stride (bf16): 80 bytes, 16-byte aligned rows
 
 // round-to-nearest-even FP32 -> BF16 bits
@@ -154,39 +152,29 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
 
-    // The K range goes in chunks of up to kChunk tiles.  All data loads of a
-    // chunk are issued at once (one memory latency per chunk, not per tile);
-    // constant fragments run two tiles ahead.
+    // One K tile at a time: its data and constant fragments are loaded, the
+    // data stored to LDS (double-buffered, so one barrier per tile suffices:
+    // every wave has finished tile t - 1 before any passes tile t's barrier and
+    // overwrites tile t - 1's buffer at t + 1) and multiplied out.  No loads are
+    // kept in flight across tiles: that keeps the kernel near 80 VGPRs, and the
+    // six waves per SIMD this allows hide the memory latency better than deeper
+    // per-wave prefetching did (the stages are short: K = 64 to 320).
     const int tiles = (g.K + kTK - 1) / kTK;
-    float2 dr[kChunk][4];
-    ConstFrag cf[3];
-    for (int c0 = 0; c0 < tiles; c0 += kChunk) {
-        const int nt = min(kChunk, tiles - c0);
-#pragma unroll
-        for (int i = 0; i < kChunk; ++i)
-            if (i < nt) load_tile<TD, kScale>(g, D, m0, (c0 + i) * kTK, dr[i]);
-        load_const(g.c, crow, c0 * kTK, cf[0]);
-        if (nt > 1) load_const(g.c, crow, (c0 + 1) * kTK, cf[1]);
-        if (c0 > 0) __syncthreads();   // the previous chunk's last tile has been read
-        store_tile(dr[0], s_hi[0], s_lo[0]);
+    for (int t = 0; t < tiles; ++t) {
+        float2 dr[4];
+        ConstFrag cf;
+        load_tile<TD, kScale>(g, D, m0, t * kTK, dr);
+        load_const(g.c, crow, t * kTK, cf);
+        store_tile(dr, s_hi[t & 1], s_lo[t & 1]);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kChunk; ++i) {
-            if (i < nt) {
-                if (i + 2 < nt) load_const(g.c, crow, (c0 + i + 2) * kTK, cf[(i + 2) % 3]);
-                const ConstFrag& cur = cf[i % 3];
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const bf16x8_t dh = *reinterpret_cast<const bf16x8_t*>(&s_hi[i & 1][drow][16 * s + kq]);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dh, cur.hi[s], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dh, cur.lo[s], acc, 0, 0, 0);
-                    if (!kExact) {
-                        const bf16x8_t dl = *reinterpret_cast<const bf16x8_t*>(&s_lo[i & 1][drow][16 * s + kq]);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dl, cur.hi[s], acc, 0, 0, 0);
-                    }
-                }
-                if (i + 1 < nt) store_tile(dr[i + 1], s_hi[(i + 1) & 1], s_lo[(i + 1) & 1]);
-                __syncthreads();
+        for (int s = 0; s < 2; ++s) {
+            const bf16x8_t dh = *reinterpret_cast<const bf16x8_t*>(&s_hi[t & 1][drow][16 * s + kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dh, cf.hi[s], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dh, cf.lo[s], acc, 0, 0, 0);
+            if (!kExact) {
+                const bf16x8_t dl = *reinterpret_cast<const bf16x8_t*>(&s_lo[t & 1][drow][16 * s + kq]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(dl, cf.hi[s], acc, 0, 0, 0);
             }
         }
     }
