@@ -583,7 +583,7 @@ template <bool kGauss>
 __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = *Dp;
     __shared__ float4 lists[4][kList];
-    __shared__ int uends[4][kList];   // a unit holds at least one slot
+    __shared__ int uends[4][kList];   // (start << 16) | end of each unit in the list
     if (blockIdx.x == gridDim.x - 1) {
         mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
@@ -652,9 +652,10 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     int fill = 0, nu = 0;   // list slots and units in use
     auto flush = [&]() {
         __builtin_amdgcn_wave_barrier();
-        int t = 0;
         for (int q = 0; q < nu; ++q) {
-            const int e = __builtin_amdgcn_readfirstlane(uends[wv][q]);
+            const int se = __builtin_amdgcn_readfirstlane(uends[wv][q]);
+            int t = se >> 16;
+            const int e = se & 0xffff;
             float acc = 0.f;
             for (; t + 2 <= e; t += 2) {
                 const float4 q0 = lists[wv][t], q1 = lists[wv][t + 1];
@@ -671,20 +672,42 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         fill = 0;
         nu = 0;
     };
+    // A charge at distance >= c from the group's box is out of cutoff for both
+    // sites of every target of the group: its terms are exactly 0 and it is not
+    // copied (leaving exact zeros out of an FP32 sum changes no bit of it, so
+    // the unit sums stay those of the whole list).
+    const float fx0 = float(x0), fx1 = float(x1), fy0 = float(y0), fy1 = float(y1);
+    const float fz0 = float(z0), fz1 = float(z1), c2f = float(c2);
+    auto near = [&](const v4f& a) {
+        const float gx = fmaxf(fmaxf(a.x - fx1, fx0 - a.x), 0.f);
+        const float gy = fmaxf(fmaxf(a.y - fy1, fy0 - a.y), 0.f);
+        const float gz = fmaxf(fmaxf(a.z - fz1, fz0 - a.z), 0.f);
+        return gx * gx + gy * gy + gz * gz < c2f;
+    };
     // copy `take` slots (whole units, or up to the brick's end) from list slot
-    // k of the charge lists to the wave's list at slot o, with their unit ends
+    // k of the charge lists to the wave's list from slot o, each unit's near
+    // charges packed at the start of its 64 slots; unit uo + j's slots are
+    // [start, end) with (start << 16) | end in the unit table
     auto copy = [&](int k, int take, int o, int uo) {
         const v4f* src = reinterpret_cast<const v4f*>(D.src) + k;
         v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]) + o;
-        int u = 0;
-        for (; u + 8 <= take; u += 8) {
-            const v4f a0 = src[u], a1 = src[u + 1], a2 = src[u + 2], a3 = src[u + 3];
-            const v4f a4 = src[u + 4], a5 = src[u + 5], a6 = src[u + 6], a7 = src[u + 7];
-            dst[u] = a0; dst[u + 1] = a1; dst[u + 2] = a2; dst[u + 3] = a3;
-            dst[u + 4] = a4; dst[u + 5] = a5; dst[u + 6] = a6; dst[u + 7] = a7;
+        for (int j = 0; j * kUnit < take; ++j) {
+            const int u0 = kUnit * j, u1 = min(take, u0 + kUnit);
+            int w = u0, u = u0;
+            for (; u + 8 <= u1; u += 8) {
+                v4f a[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) a[i] = src[u + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (near(a[i])) dst[w++] = a[i];
+            }
+            for (; u < u1; ++u) {
+                const v4f a = src[u];
+                if (near(a)) dst[w++] = a;
+            }
+            uends[wv][uo + j] = ((o + u0) << 16) | (o + w);
         }
-        for (; u < take; ++u) dst[u] = src[u];
-        for (int j = 0; j * kUnit < take; ++j) uends[wv][uo + j] = o + min(kUnit * (j + 1), take);
     };
     // one brick segment per lane, [k, k + todo), in rounds of whole units
     auto segment = [&](int k, int todo) {
