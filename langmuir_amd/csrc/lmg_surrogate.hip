@@ -274,35 +274,10 @@ __global__ void occ_clear_kernel(uint32_t* __restrict__ occ, long long words, in
         occ[i] = 0u;
 }
 
-// occ_scatter_kernel plus the (x, y, z, q) coordinates of every charge for the
-// exact pair sums (the work of to_float4_kernel), in one pass over the charges.
-__global__ void occ_scatter_f4_kernel(const int32_t* __restrict__ site, const int32_t* __restrict__ q, int n,
-                                      lmg::Dims d, int8_t* occ, int* bad, float4* __restrict__ chg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = site[i];
-    const int qi = q[i];
-    if (s < 0 || s >= d.V || qi < -127 || qi > 127) {
-        atomicOr(bad, 1);
-        chg[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
-    }
-    int x, y, z;
-    lmg::decode(d, s, x, y, z);
-    chg[i] = make_float4(float(x), float(y), float(z), float(qi));
-    unsigned int* w = reinterpret_cast<unsigned int*>(occ + (s & ~3));
-    const int sh = (s & 3) * 8;
-    unsigned int old = *w, assumed;
-    do {
-        assumed = old;
-        const int b = int(int8_t((assumed >> sh) & 0xffu)) + qi;
-        const unsigned int nv = (assumed & ~(0xffu << sh)) | (unsigned(uint8_t(int8_t(b))) << sh);
-        old = atomicCAS(w, assumed, nv);
-    } while (old != assumed);
-}
-
-// occ_scatter_f4_kernel for a batch of charge lists: charge i belongs to list b
-// with off[b] <= i < off[b+1] and goes into grid occ + b * V.
+// occ_scatter_kernel for a batch of charge lists, plus the (x, y, z, q)
+// coordinates of every charge for the exact pair sums (the work of
+// to_float4_kernel) in the same pass: charge i belongs to list b with
+// off[b] <= i < off[b+1] and goes into grid occ + b * V.
 __global__ void occ_scatter_batch_kernel(const int32_t* __restrict__ site, const int32_t* __restrict__ q,
                                          const int* __restrict__ off, int batch, lmg::Dims d, int8_t* occ, int* bad,
                                          float4* __restrict__ chg) {
@@ -323,9 +298,11 @@ __global__ void occ_scatter_batch_kernel(const int32_t* __restrict__ site, const
     int x, y, z;
     lmg::decode(d, s, x, y, z);
     chg[i] = make_float4(float(x), float(y), float(z), float(qi));
-    int8_t* g = occ + (long long)lo * d.V;
-    unsigned int* w = reinterpret_cast<unsigned int*>(g + (s & ~3));
-    const int sh = (s & 3) * 8;
+    // grid lo starts at byte lo V, which need not be 4-byte aligned: the word
+    // holding the site's byte is found from its absolute offset
+    const long long a = (long long)lo * d.V + s;
+    unsigned int* w = reinterpret_cast<unsigned int*>(occ + (a & ~3ll));
+    const int sh = int(a & 3) * 8;
     unsigned int old = *w, assumed;
     do {
         assumed = old;
