@@ -171,6 +171,24 @@ __device__ long long wg_prefix(const int* arr, int nb, int stride, int off, long
     return *slot;
 }
 
+// wg_prefix in two halves, so that a kernel can issue its per-carrier loads
+// between them: the first wave's lane sums of arr[stride * b + off], b < nb
+// (no barrier), then their total, shared with the workgroup (a barrier)
+__device__ __forceinline__ long long wg_prefix_loads(const int* arr, int nb, int stride, int off) {
+    long long s = 0;
+    if (threadIdx.x < 64)
+        for (int b = threadIdx.x; b < nb; b += 64) s += arr[stride * b + off];
+    return s;
+}
+__device__ __forceinline__ long long wg_prefix_total(long long s, long long* slot) {
+    if (threadIdx.x < 64) {
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (threadIdx.x == 0) *slot = s;
+    }
+    __syncthreads();
+    return *slot;
+}
+
 // ---------------------------------------------------------------------------
 // MT19937 refill: x[m] = x[m-227] ^ twist(x[m-624], x[m-623]), so 227
 // consecutive words can be produced at once from the last 624.
@@ -831,10 +849,23 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     const bool isE = i < ne;
     const int li = isE ? i : i - ne;
     const int q = isE ? -1 : 1;
-    int site = 0, fs = 0;
+    // draws of the earlier workgroups: loads issued here, summed after the
+    // carrier's own loads
+    const long long draws_lane = only_de ? 0 : wg_prefix_loads(D.blk_draw, blockIdx.x, 1, 0);
+    int site = 0, fs = 0, draw = 0, life = 0, path = 0;
+    uint8_t tf = kEmpty;
+    double vb_fs = 0.0, vb_site = 0.0;
     if (valid) {
         site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
         fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+        // the decision's inputs (fs may be a drain slot: V + k is inside the
+        // type and background grids)
+        draw = D.flag[i];
+        tf = isE ? D.etype[fs] : D.htype[fs];
+        vb_fs = D.vbg[fs];
+        vb_site = D.vbg[site];
+        life = isE ? D.e_life[cur][li] : D.h_life[cur][li];
+        path = isE ? D.e_path[cur][li] : D.h_path[cur][li];
         if (D.cull && !only_de) {   // the step is done with this carrier's bitmask word and brick counters
             int x, y, z;
             decode(D.d, site, x, y, z);
@@ -877,17 +908,15 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     if (only_de) return;
 
     if (threadIdx.x < 4) { att[threadIdx.x] = 0; suc[threadIdx.x] = 0; }
-    const long long before = wg_prefix(D.blk_draw, blockIdx.x, 1, 0, &pre);
+    const long long before = wg_prefix_total(draws_lane, &pre);
     int total;
-    const int draw = valid ? D.flag[i] : 0;
     const int off = wg_scan<kSimBlock>(draw, wsum, total);
     int code = 0, keep = 1;
     if (valid) {
-        const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
         if (draw) {
             const double u = u01(rng_word(D, C.rng_pos + (unsigned long long)(before + off)));
             if (tf == kEmpty) {
-                double pd = D.vbg[fs] - D.vbg[site];
+                double pd = vb_fs - vb_site;
                 pd *= q;
                 pd += de;
                 int x1, y1, z1, x2, y2, z2;
@@ -910,10 +939,8 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
                 }
             }
         }
-        int* life = isE ? D.e_life[cur] : D.h_life[cur];
-        int* path = isE ? D.e_path[cur] : D.h_path[cur];
-        life[li] += 1;
-        if (code) path[li] += 1;
+        (isE ? D.e_life[cur] : D.h_life[cur])[li] = life + 1;
+        if (code) (isE ? D.e_path[cur] : D.h_path[cur])[li] = path + 1;
         else fs = site;
         if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
         D.code[i] = code;
@@ -1085,13 +1112,17 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
     const bool valid = i < N;
     const bool isE = i < ne;
     const int li = isE ? i : i - ne;
-    const long long base_e = wg_prefix(D.blk_keep, blockIdx.x, 2, 0, &pre_e);
-    const long long base_h = wg_prefix(D.blk_keep, blockIdx.x, 2, 1, &pre_h);
-    int keep_e = 0, keep_h = 0, dest = 0;
+    // kept carriers of the earlier workgroups: loads issued here, summed
+    // after the carrier's own work
+    const long long keep_e_lane = wg_prefix_loads(D.blk_keep, blockIdx.x, 2, 0);
+    const long long keep_h_lane = wg_prefix_loads(D.blk_keep, blockIdx.x, 2, 1);
+    int keep_e = 0, keep_h = 0, dest = 0, life = 0, path = 0;
     if (valid) {
         const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
         const int code = D.code[i];
         const bool rem = D.removed[i] != 0;
+        life = isE ? D.e_life[cur][li] : D.h_life[cur][li];
+        path = isE ? D.e_path[cur][li] : D.h_path[cur][li];
         uint8_t* ty = isE ? D.etype : D.htype;
         dest = site;
         if (rem || code == 2) {
@@ -1110,19 +1141,21 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
         keep_e = isE && keep;
         keep_h = !isE && keep;
     }
+    const long long base_e = wg_prefix_total(keep_e_lane, &pre_e);
+    const long long base_h = wg_prefix_total(keep_h_lane, &pre_h);
     int te, th;
     const int oe = wg_scan<kSimBlock>(keep_e, wsum, te);
     const int oh = wg_scan<kSimBlock>(keep_h, wsum, th);
     if (keep_e) {
         const int w = int(base_e) + oe;
         D.e_site[nxt][w] = dest;
-        D.e_life[nxt][w] = D.e_life[cur][li];
-        D.e_path[nxt][w] = D.e_path[cur][li];
+        D.e_life[nxt][w] = life;
+        D.e_path[nxt][w] = path;
     } else if (keep_h) {
         const int w = int(base_h) + oh;
         D.h_site[nxt][w] = dest;
-        D.h_life[nxt][w] = D.h_life[cur][li];
-        D.h_path[nxt][w] = D.h_path[cur][li];
+        D.h_life[nxt][w] = life;
+        D.h_path[nxt][w] = path;
     }
 }
 
