@@ -130,6 +130,31 @@ def test_transistor_full_size_parity(gpu, oracle, de_path):
     assert mism <= 2
 
 
+@pytest.mark.parametrize("name,over", [
+    # axes longer than twice the cutoff: x only (bricks ordered x-major, y and z fastest)
+    ("defects", dict(electrostatic_cutoff=10)),
+    # x and y (a Hilbert curve over 5 x 3 bricks, inside an 8 x 8 square)
+    ("defects", dict(grid_x=40, grid_y=24, electrostatic_cutoff=8)),
+    # all three (Morton order)
+    ("solar3d", dict(grid_x=24, grid_y=24, grid_z=24, electrostatic_cutoff=8)),
+])
+def test_culled_brick_orders(gpu, oracle, name, over):
+    """The culled dE path orders its bricks, and so its target groups, by the
+    lattice's shape against the cutoff; every order gives the oracle's dE and
+    trajectory."""
+    w, sim, sp = make(oracle, gpu, name, "culled", **over)
+    fe, fh, de, dh = sim.delta_e()
+    w.propose()
+    w.coulomb()
+    e, h = w.carriers(0), w.carriers(1)
+    assert np.array_equal(fe, e["fsite"]) and np.array_equal(fh, h["fsite"])
+    for got, ref in ((de, e["de"]), (dh, h["de"])):
+        assert np.all(np.abs(got - ref) <= de_tolerance(ref))
+    w, sim, sp = make(oracle, gpu, name, "culled", **over)
+    assert run_parity(w, sim, 40, sp.inverse_kt) <= 2
+    assert sim.device_error() == 0
+
+
 def test_solar_full_size_delta_e(gpu, oracle):
     """128x128x32 with 5242 electrons + 5242 holes, cutoff 30: proposals bit-exact,
     dE within tolerance for all 10,484 carriers."""
