@@ -159,21 +159,10 @@ __device__ int wg_scan(int v, int* wsum, int& total) {
     return excl;
 }
 
-// sum of arr[stride*b + off] over b < nb, computed by wave 0 and broadcast
-__device__ long long wg_prefix(const int* arr, int nb, int stride, int off, long long* slot) {
-    if (threadIdx.x < 64) {
-        long long s = 0;
-        for (int b = threadIdx.x; b < nb; b += 64) s += arr[stride * b + off];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (threadIdx.x == 0) *slot = s;
-    }
-    __syncthreads();
-    return *slot;
-}
-
-// wg_prefix in two halves, so that a kernel can issue its per-carrier loads
-// between them: the first wave's lane sums of arr[stride * b + off], b < nb
-// (no barrier), then their total, shared with the workgroup (a barrier)
+// Workgroup prefix over per-workgroup counts, sum of arr[stride * b + off]
+// for b < nb, in two halves so that a kernel can issue its per-carrier loads
+// between them: the first wave's lane sums (loads only, no barrier), then
+// their total, shared with the workgroup (a barrier)
 __device__ __forceinline__ long long wg_prefix_loads(const int* arr, int nb, int stride, int off) {
     long long s = 0;
     if (threadIdx.x < 64)
