@@ -182,8 +182,11 @@ lmg_status lmg_sim_set_graphs(lmg_ctx* ctx, int32_t on);
  * (sized from the occupancy query) and *kernel_us = -1.  The sums do not
  * depend on either.  No counterpart in the reference. */
 lmg_status lmg_sim_get_tuning(const lmg_ctx* ctx, int32_t* shape, double* kernel_us);
-/* Fixes the all-pairs ΔE kernel's charge-chunk count for the following loads
- * (1..128) instead of autotuning it (0, the default). */
+/* Fixes the ΔE kernel's shape for the following loads instead of choosing it
+ * (0, the default): on the all-pairs path the charge-chunk count (1..128), on
+ * the culled path the number of workgroups (at least one per target group;
+ * at most 65536).  The sums do not depend on it.  No counterpart in the
+ * reference. */
 lmg_status lmg_sim_set_de_chunks(lmg_ctx* ctx, int32_t chunks);
 /* dE path for the following lmg_sim_load: 0 = automatic (culled when a group
  * of targets plus the cutoff covers well under the lattice and the pair count

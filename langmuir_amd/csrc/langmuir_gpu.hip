@@ -694,7 +694,9 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         // work items (workgroups of four chunks of one target group): as many
         // as keep every workgroup of cull_de_kernel resident at once (one wave
         // of workgroups, no tail), at least one per group; items_cap counts waves
-        D.items_cap = 4 * std::max(ncu * per_cu - 1, D.ngrp);
+        // (lmg_sim_set_de_chunks fixes the workgroup count instead)
+        const int wgs = c->forced_chunks > 0 ? std::min(c->forced_chunks, kMaxCullWorkgroups) : ncu * per_cu - 1;
+        D.items_cap = 4 * std::max(wgs, D.ngrp);
     }
 
     for (int b = 0; b < 2; ++b) {

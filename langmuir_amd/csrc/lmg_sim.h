@@ -24,6 +24,7 @@ namespace lmg {
 
 constexpr int kSimBlock = 256;   // carriers per workgroup in the grid phases
 constexpr int kOneBlock = 1024;  // single-workgroup phases
+constexpr int kMaxCullWorkgroups = 65536;   // cull_de_kernel workgroups a caller may ask for
 
 // device error codes (lmg_sim_device_error)
 constexpr int kErrRing = 1, kErrCapacity = 2, kErrSite = 3, kErrBalance = 4;

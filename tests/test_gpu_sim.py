@@ -240,6 +240,26 @@ def test_all_pairs_de_independent_of_chunking(gpu, oracle, name):
         assert np.array_equal(de, res[0][0]) and np.array_equal(dh, res[0][1])
 
 
+@pytest.mark.parametrize("name", ["solar3d", "defects"])
+def test_culled_de_independent_of_workgroups(gpu, oracle, name):
+    """The culled dE kernel adds exact fixed-point sums of the same list units
+    whatever its workgroup count, so a forced shape changes no bit."""
+    op, sp = params_pair(**CONFIGS[name])
+    w = oracle.World(op)
+    res = []
+    for wgs in (1, 5, 300, 0):
+        sim = gpu_from_oracle(gpu, w, sp, de_path="culled", de_chunks=wgs)
+        res.append(sim.delta_e()[2:])
+        t = sim.tuning
+        assert t["path"] == "culled"
+        if wgs:   # four work items (chunks) per workgroup; one workgroup per target group at least
+            assert t["work_items"] >= 4 * wgs and t["work_items"] % 4 == 0
+        if wgs == 300:   # more than these lattices have target groups
+            assert t["work_items"] == 1200
+    for de, dh in res[1:]:
+        assert np.array_equal(de, res[0][0]) and np.array_equal(dh, res[0][1])
+
+
 def test_load_rejects_bad_state(gpu):
     sp = gpu.SimulationParameters(grid_x=8, grid_y=8, grid_z=1, coulomb_carriers=True,
                                   electron_percentage=0.0, hole_percentage=0.1,
