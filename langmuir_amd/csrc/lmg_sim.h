@@ -69,8 +69,8 @@ struct SimDev {
     int* tcnt;                    // targets per brick (same)
     int2* tpos;                   // per target: its brick, its slot in the brick
     int* sstart;                  // [nbrick + 1] charge list offsets
-    int* tstart;                  // [nbrick] first target slot of each brick (bricks in Morton order)
-    const int* bperm;             // bricks in Morton order (static)
+    int* tstart;                  // [nbrick] first target slot of each brick (bricks in bperm order)
+    const int* bperm;             // brick order of the target groups (static, set at load)
     const int* dstart;            // [nbrick + 1] static charged-defect list offsets
     const float4* dsrc;           // static charged defects, by brick then site
     float4* src;                  // charge lists of the bricks
@@ -315,8 +315,10 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 // The lattice is cut into bricks of 8 x 8 x 8 sites (partial at the far faces).
 // Every step the charges are listed brick by brick in a fixed order -- in a
 // brick: electrons, holes (each by site, from occupancy bitmasks), then the
-// static charged defects.  Targets are grouped 64 to a wave in Morton order of
-// their bricks, so a group's current and proposed sites fill a small box.
+// static charged defects.  Targets are grouped 64 to a wave in the order of
+// their bricks (bperm: the lattice's thin axes fastest, a Hilbert curve or
+// Morton order over the long ones), so a group's current and proposed sites
+// fill a small box.
 //
 //   cull_mark (propose_kernel, or cull_mark_kernel after serial proposals):
 //     each carrier's bitmask bit, per-brick charge and target counts, and the
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_mark_kernel(const SimDev* __re
     cull_mark(D, i, isE, isE ? D.e_site[cur][i] : D.h_site[cur][i - ne]);
 }
 
-// Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the targets (bricks in Morton order; tst[brick])
+// Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the targets (bricks in bperm order; tst[brick])
 // by one workgroup of kThreads; thread t scans a contiguous range of bricks,
 // its first kPer counts loaded together.
 template <int kThreads>
