@@ -296,11 +296,11 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
 lmg_status sim_launch_propose(lmg_ctx* c) {
     const SimDev& D = c->sim;
     // culled dE: propose_kernel also marks the carriers' bricks (cull_mark_kernel
-    // after serial proposals) and a kOneBlock propose_fix_kernel (cull_scan_kernel)
-    // turns the counts into offsets
+    // after serial proposals) and propose_fix_kernel, one workgroup of kSimBlock
+    // (cull_scan_kernel), turns the counts into offsets
     if (D.uniform) {
         hipLaunchKernelGGL(propose_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
-        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(D.cull ? kOneBlock : 64), 0, c->stream, c->d_sim);
+        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(D.cull ? kSimBlock : 64), 0, c->stream, c->d_sim);
     } else if (D.cull) {
         hipLaunchKernelGGL(cull_mark_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
         hipLaunchKernelGGL(cull_scan_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim);

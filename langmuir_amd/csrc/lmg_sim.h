@@ -1438,12 +1438,12 @@ __global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __rest
 
 // After propose_kernel: redo serially from the first rejection (if any) and
 // set the engine position of the pending decisions.
-__global__ __launch_bounds__(kOneBlock) void propose_fix_kernel(const SimDev* __restrict__ Dp) {
+__global__ __launch_bounds__(kSimBlock) void propose_fix_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];   // replica
     SimCtl& C = *D.ctl;
-    if (D.cull) {   // one workgroup of kOneBlock: the culled dE's offsets first
-        __shared__ int wsum[kOneBlock / 64];
-        cull_offsets<kOneBlock>(D, D.sstart, D.tstart, wsum);
+    if (D.cull) {   // one workgroup of kSimBlock: the culled dE's offsets first
+        __shared__ int wsum[kSimBlock / 64];
+        cull_offsets<kSimBlock>(D, D.sstart, D.tstart, wsum);
     }
     if (threadIdx.x != 0 || !D.uniform) return;
     const int ne = C.n_e, N = ne + C.n_h;
