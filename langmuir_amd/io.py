@@ -11,10 +11,14 @@
   (writer.cpp:545-581), every site written (the reference's read-back wrote
   only n values; see lmg_potential_map).
 
-``[RandomState]`` is the text state of Boost's mt19937 (rand.cpp:359-381): 624
-raw words, oldest first, which is exactly the engine window ``rng_window`` of
-this package's simulation state, so a checkpoint resumes the same random
-stream.
+``[RandomState]`` is what ``operator<<(std::ostream&, Random&)`` writes
+(rand.cpp:359-364; read back in that order by rand.cpp:366-381): the seed
+``m_seed``, then the text state of Boost's mt19937.  That text is the C++
+standard's representation of the engine, the last 624 raw words X(i-624) ..
+X(i-1), oldest first (Boost >= 1.47 rewinds its in-place block to print them;
+1.44's ``compute(j)`` prints the same words with a trailing blank).  Those words
+are exactly the engine window ``rng_window`` of this package's simulation state,
+so a checkpoint resumes the same random stream.
 """
 from __future__ import annotations
 
@@ -310,6 +314,7 @@ class Checkpoint:
     trap_potentials: Optional[List[float]] = None
     flux: Optional[List[int]] = None          # attempts and successes, interleaved per agent
     random_state: Optional[List[int]] = None  # 624 raw mt19937 words, oldest first
+    random_seed: Optional[int] = None         # Random::m_seed, written before the words
 
 
 def _open_text(path: str) -> str:
@@ -431,6 +436,8 @@ def parse_checkpoint(text: str) -> Checkpoint:
         elif section == "FluxState":
             ck.flux = read_values("flux", u64, read_count("fluxes"))
         elif section == "RandomState":
+            # stream >> m_seed, then stream >> *twister (rand.cpp:366-381)
+            ck.random_seed = read_values("random number generator seed", u64, 1)[0]
             ck.random_state = read_values("random number generator state word", uint, 624)
     return ck
 
@@ -451,9 +458,12 @@ def format_checkpoint(ck: Checkpoint, trap_potentials: Optional[bool] = None) ->
         parts.append(f"[TrapPotentials]\n{len(tp)}" + "".join(f"\n{float(v):.17e}" for v in tp))
     parts.append(ints("FluxState", [0] * (2 * len(FLUX_AGENT_NAMES)) if ck.flux is None else ck.flux))
     words = ck.random_state
-    if words is None:
-        raise ValueError("a checkpoint carries the engine state ([RandomState])")
-    parts.append("[RandomState]\n" + " ".join(str(int(w)) for w in words))
+    if words is None or len(words) != 624:
+        raise ValueError("a checkpoint carries the engine state ([RandomState], 624 words)")
+    if ck.random_seed is None:
+        raise ValueError("a checkpoint carries the engine's seed (Random::m_seed) before its words")
+    # stream << m_seed << ' ' << *twister (rand.cpp:359-364)
+    parts.append(f"[RandomState]\n{int(ck.random_seed)} " + " ".join(str(int(w)) for w in words))
     return "\n".join(parts) + "\n[Parameters]" + ck.parameters.format()
 
 
@@ -489,7 +499,8 @@ def setup_of(ck: Checkpoint):
     ck.parameters.check()
     return initialize(ck.parameters.to_parameters(), electrons=ck.electrons or (), holes=ck.holes or (),
                       defects=ck.defects or (), traps=ck.traps or (),
-                      trap_potentials=ck.trap_potentials or None, rng_window=ck.random_state)
+                      trap_potentials=ck.trap_potentials or None, rng_window=ck.random_state,
+                      rng_seed=ck.random_seed)
 
 
 def simulation_kwargs(ck: Checkpoint, setup) -> dict:
@@ -521,7 +532,8 @@ def checkpoint_of(state: dict, parameters: KeyValues, setup) -> Checkpoint:
     return Checkpoint(parameters=kv, electrons=[int(v) for v in state["e_site"]],
                       holes=[int(v) for v in state["h_site"]], defects=list(setup.defects),
                       traps=list(setup.traps), trap_potentials=list(setup.trap_potentials),
-                      flux=flux, random_state=[int(w) for w in np.asarray(state["rng_window"], np.uint64)])
+                      flux=flux, random_state=[int(w) for w in np.asarray(state["rng_window"], np.uint64)],
+                      random_seed=int(setup.rng_seed))
 
 
 # ---------------------------------------------------------------------------

@@ -16,12 +16,14 @@ an input or checkpoint file lists) are placed first; the rest is drawn:
   hopping-range-1 neighbour (potential.cpp:170-220).
 
 uniform_int is Boost's (>= 1.47) bucket rejection on 32-bit words (no engine
-call for a one-value range).  Trap depths from ``gaussian.stdev > 0`` draw from
-boost::normal_distribution, whose algorithm changed across Boost versions; that
-case is refused (give explicit [Traps]/[TrapPotentials] instead).
+call for a one-value range).  With ``gaussian.stdev > 0`` every drawn trap's
+depth gets a boost::normal_distribution deviation (potential.cpp:211-220), drawn
+as Boost 1.44-1.55 draw it (Box-Muller, two words per value); Boost 1.56
+replaced that algorithm, so runs under a newer Boost draw other depths.
 """
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
@@ -101,6 +103,19 @@ class MT19937:
             if r <= rng:
                 return lo + r
 
+    def random(self) -> float:
+        """Random::random (rand.cpp:49-52): boost::uniform_01<double>, one word times 2^-32."""
+        return self() * (1.0 / 4294967296.0)
+
+    def normal(self, mean: float, sigma: float) -> float:
+        """Random::normal (rand.cpp:61-66): a fresh boost::normal_distribution per call.  In
+        Boost 1.44-1.55, the versions the reference's build files name, that is Box-Muller
+        over two uniform_01 draws r1, r2, and a fresh distribution returns the cosine
+        branch: sqrt(-2 log(1 - r2)) cos(2 pi r1) sigma + mean."""
+        r1 = self.random()
+        r2 = self.random()
+        return math.sqrt(-2.0 * math.log(1.0 - r2)) * math.cos(2.0 * 3.14159265358979323846 * r1) * sigma + mean
+
 
 def _neighbors_h1(dims, s: int) -> List[int]:
     """Grid::neighborsSite(s, 1) of the electron grid: -x, +x, -y, +y, -z, +z
@@ -141,17 +156,20 @@ class WorldSetup:
     potential: np.ndarray        # background potential [V]: linear + gate + traps
     rng_window: np.ndarray       # engine window after the placement draws
     rng_consumed: int            # outputs drawn during set-up
+    rng_seed: int = 0            # Random::m_seed: the [RandomState] seed, else uint32(random.seed)
 
 
 def initialize(p: SimulationParameters, electrons: Sequence[int] = (), holes: Sequence[int] = (),
                defects: Sequence[int] = (), traps: Sequence[int] = (),
                trap_potentials: Optional[Sequence[float]] = None,
-               rng_window: Optional[Sequence[int]] = None) -> WorldSetup:
+               rng_window: Optional[Sequence[int]] = None, rng_seed: Optional[int] = None) -> WorldSetup:
     """World::initialize's placement for explicit lists plus the seeded rest.
 
     The engine starts from ``rng_window`` (a [RandomState]) when given, else from
     ``random.seed`` (0 seeds from the clock, as Random(0) does, rand.cpp:8-20; the
-    seed used is written back into the returned parameters)."""
+    seed used is written back into the returned parameters).  The set-up keeps the
+    engine's seed as Random::m_seed holds it: ``rng_seed`` (the seed a [RandomState]
+    carried) with a window, else ``random.seed`` cast to 32 bits (rand.cpp:36-47)."""
     if rng_window is None and p.random_seed == 0:
         p = SimulationParameters(**{**p.__dict__, "random_seed": int(time.time()) & 0xFFFFFFFF})
     p.check()
@@ -251,9 +269,6 @@ def initialize(p: SimulationParameters, electrons: Sequence[int] = (), holes: Se
     t_list = list(forced)
     t_pot = tp_in if tp_in else [p.trap_potential] * len(forced)
     if randomly > 0:
-        if abs(p.gaussian_stdev) > 0:
-            raise NotImplementedError("gaussian.stdev > 0 draws trap depths from boost::normal_distribution; "
-                                      "list the traps and their potentials explicitly instead")
         forced_set = set(forced)
         rids: List[int] = []
         rset = set()
@@ -271,10 +286,16 @@ def initialize(p: SimulationParameters, electrons: Sequence[int] = (), holes: Se
                 rids.append(ns)
                 rset.add(ns)
                 progress += 1
+        r_pot = [p.trap_potential] * len(rids)
+        # a normal deviation per randomly placed trap, in list order (potential.cpp:211-220)
+        if abs(p.gaussian_stdev) > 0:
+            r_pot = [v + rng.normal(0.0, p.gaussian_stdev) for v in r_pot]
         t_list += rids
-        t_pot += [p.trap_potential] * len(rids)
+        t_pot += r_pot
 
     pot = background_potential(p, t_list, t_pot if t_list else None)
     return WorldSetup(params=p, electrons=e_list, holes=h_list, defects=d_list, traps=t_list,
                       trap_potentials=t_pot, potential=pot, rng_window=rng.window(),
-                      rng_consumed=rng.consumed)
+                      rng_consumed=rng.consumed,
+                      rng_seed=(int(rng_seed) if rng_window is not None and rng_seed is not None
+                                else int(p.random_seed) & 0xFFFFFFFF))

@@ -133,6 +133,17 @@ struct Random {
         if (m_out) *m_out = lhs - u;
         return lhs > u;
     }
+    /* :61-66, a fresh boost::normal_distribution per call.  Boost 1.44-1.55 (the versions
+     * the reference's build files name: src/Makefile:22, src/compile_frank:10-11) draw it by
+     * Box-Muller from two uniform_01 draws; a fresh distribution always takes the cosine
+     * branch: rho = sqrt(-2 log(1 - r2)), value = rho cos(2 pi r1) sigma + mean.  (Boost
+     * >= 1.56 switched to a ziggurat: parity unpinned across that change.) */
+    double normal(double mean, double sigma) {
+        double r1 = random();
+        double r2 = random();
+        double rho = std::sqrt(-2.0 * std::log(1.0 - r2));
+        return rho * std::cos(2.0 * 3.14159265358979323846 * r1) * sigma + mean;
+    }
     bool chooseYes(double p, double* u_out = nullptr, double* m_out = nullptr) {  // :117-124
         double u = random();
         if (u_out) *u_out = u;
@@ -802,8 +813,9 @@ void set_potential_traps(orc_world& W, const std::vector<int>& ids, const std::v
                 rids.push_back(ns); rpots.push_back(W.p().trap_potential); ++progress;
             }
         }
+        // potential.cpp:211-220: a normal deviation per randomly placed trap, in list order
         if (std::abs(W.p().gaussian_stdev) > 0)
-            fail("gaussian.stdev > 0 draws from boost::normal_distribution (not restated)");
+            for (size_t i = 0; i < rids.size(); ++i) rpots[i] += W.rng.normal(0.0, W.p().gaussian_stdev);
         for (size_t i = 0; i < rids.size(); ++i) { traps.push_back(rids[i]); potentials.push_back(rpots[i]); }
     }
     W.traps = traps;

@@ -65,6 +65,7 @@ def test_run_input_outputs(gpu, oracle, tmp_path):
     ck = L.read_checkpoint(str(out / "run.chk"))
     assert ck.holes == state["h_site"].tolist() and ck.parameters["current.step"] == 40
     assert ck.random_state == state["rng_window"].tolist()
+    assert ck.random_seed == 17      # Random::m_seed, written before the twister's words
     assert L.read_checkpoint(str(out / "run.parm")).parameters["output.stub"] == "run"
     # the map at step 40: every site, the carriers of that step
     s, v = L.read_coulomb(str(out / "run-40.coulomb"))
@@ -134,3 +135,43 @@ def test_reference_example_inputs_step_like_the_oracle(gpu, oracle, name, seed):
     assert sim.device_error() == 0
     if name == "transistor.inp":   # the left electron source injects at 0.9 per step
         assert len(sim.state()["e_site"]) > 0
+
+
+@pytest.mark.parametrize("seed,blocks", [(31337, 2), (4000000000, 5)])
+def test_reference_random_state_resumes_on_the_device(gpu, oracle, seed, blocks):
+    """The reference's example transistor input with a [RandomState] of 625 integers in
+    the reference's layout (m_seed, then the 624 words of the twister's text state,
+    rand.cpp:359-381), built from numpy's legacy MT19937 after `blocks` whole blocks of
+    draws: the file's stream (not random.seed) places the carriers, and the device
+    steps from it exactly as the oracle does from the same window."""
+    from test_gpu_sim import run_parity
+    rs = np.random.RandomState(seed)
+    rs.randint(0, 2 ** 32, size=624 * blocks, dtype=np.uint64)
+    key, pos = rs.get_state()[1:3]
+    assert pos == 624
+    section = f"[RandomState]\n{seed} " + " ".join(str(int(w)) for w in key) + "\n"
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ref_files", "transistor.inp")) as f:
+        text = f.read()
+    ck = L.parse_checkpoint(section + text)
+    ck.parameters["random.seed"] = 5           # ignored: the file carries the engine
+    assert ck.random_seed == seed and ck.random_state == [int(w) for w in key]
+    setup = L.setup_of(ck)
+    assert setup.rng_seed == seed
+    # the first engine draws of the set-up are numpy's next outputs from that state
+    from langmuir_amd import world as W
+    mt = W.MT19937(window=ck.random_state)
+    assert [mt() for _ in range(64)] == [int(v) for v in rs.randint(0, 2 ** 32, size=64, dtype=np.uint64)]
+    sp = setup.params
+    print(f"reference [RandomState] seed {seed} after {blocks} blocks: {len(setup.holes)} h", flush=True)
+    w = oracle.World(oracle_params(oracle, sp), electrons=setup.electrons, holes=setup.holes,
+                     defects=setup.defects, traps=setup.traps, trap_potentials=setup.trap_potentials)
+    w.set_rng(setup.rng_window, setup.rng_consumed)
+    sim = gpu.Simulation(sp, **L.simulation_kwargs(ck, setup))
+    sim.set_synchronous(True)
+    assert run_parity(w, sim, 30, sp.inverse_kt) <= 2
+    assert sim.device_error() == 0
+    # the checkpoint written from the device state carries the file's seed
+    out = L.format_checkpoint(L.checkpoint_of(sim.state(), ck.parameters, setup))
+    words = out.split("[RandomState]\n")[1].split("\n")[0].split()
+    assert len(words) == 625 and int(words[0]) == seed
+    assert [int(x) for x in words[1:]] == sim.state()["rng_window"].tolist()

@@ -99,6 +99,9 @@ def test_section_errors():
         L.parse_checkpoint("[Holes]\n1\n-4\n")
     with pytest.raises(L.ParseError):
         L.parse_checkpoint("[RandomState]\n" + " ".join(["1"] * 623) + "\n")
+    # the seed comes first (rand.cpp:366-381): 624 integers leave the twister one word short
+    with pytest.raises(L.ParseError):
+        L.parse_checkpoint("[RandomState]\n" + " ".join(["1"] * 624) + "\n[Parameters]\n")
 
 
 # ----------------------------------------------------------------------------- writing
@@ -130,17 +133,18 @@ def test_checkpoint_round_trip(tmp_path):
         kv.parse_line(line)
     ck = L.Checkpoint(parameters=kv, electrons=[3, 1, 99], holes=[], defects=[5],
                       traps=[7, 8], trap_potentials=[0.1, 1 / 3], flux=list(range(20)),
-                      random_state=[int(w) for w in rng.integers(0, 2 ** 32, 624)])
+                      random_state=[int(w) for w in rng.integers(0, 2 ** 32, 624)], random_seed=424242)
     text = L.format_checkpoint(ck)
     assert text.startswith("[Electrons]\n3\n3\n1\n99\n[Holes]\n0\n[Defects]\n1\n5\n[Traps]\n2\n7\n8\n"
                            "[TrapPotentials]\n2\n1.00000000000000006e-01\n3.33333333333333315e-01\n"
                            "[FluxState]\n20\n0\n1\n")
-    assert "\n[RandomState]\n" + " ".join(map(str, ck.random_state)) + "\n[Parameters]\n" in text
+    assert "\n[RandomState]\n424242 " + " ".join(map(str, ck.random_state)) + "\n[Parameters]\n" in text
     assert not text.endswith("\n")
     path = tmp_path / "a.chk"
     L.write_checkpoint(str(path), ck)
     back = L.read_checkpoint(str(path))
-    for f in ("electrons", "holes", "defects", "traps", "trap_potentials", "flux", "random_state"):
+    for f in ("electrons", "holes", "defects", "traps", "trap_potentials", "flux", "random_state",
+              "random_seed"):
         assert getattr(back, f) == getattr(ck, f), f
     # max.threads is the last line and has no newline after it: the reference drops it too
     assert {k: v for k, v in back.parameters.values.items() if k != "max.threads"} == \
@@ -234,6 +238,9 @@ WORLDS = {
     "defects_and_traps": dict(grid_x=24, grid_y=24, grid_z=2, electron_percentage=0.04, seed_charges=0.5,
                               defect_percentage=0.03, trap_percentage=0.1, seed_percentage=1.0,
                               coulomb_carriers=1, defects_charge=-1, random_seed=901),
+    "gaussian_trap_depths": dict(grid_x=16, grid_y=12, grid_z=3, electron_percentage=0.03, seed_charges=1.0,
+                                 trap_percentage=0.15, seed_percentage=0.3, trap_potential=0.3,
+                                 gaussian_stdev=0.1, random_seed=4242),
 }
 
 
@@ -270,10 +277,6 @@ def test_world_setup_with_listed_sites(oracle):
 
 
 def test_world_setup_refusals():
-    _, sp = params_pair(grid_x=8, grid_y=8, electron_percentage=0.05, trap_percentage=0.2, gaussian_stdev=0.1,
-                        random_seed=2)
-    with pytest.raises(NotImplementedError):
-        W.initialize(sp)
     _, sp = params_pair(grid_x=8, grid_y=8, electron_percentage=0.05, random_seed=2)
     with pytest.raises(ValueError):
         W.initialize(sp, electrons=[1, 2, 3, 4])          # int(0.05 * 64) = 3 allowed
@@ -294,8 +297,9 @@ def test_setup_of_checkpoint_resumes_engine():
     mt = W.MT19937(1234)
     for _ in range(1000):
         mt()
-    ck = L.Checkpoint(parameters=kv, electrons=[0, 1], random_state=mt.window().tolist())
+    ck = L.Checkpoint(parameters=kv, electrons=[0, 1], random_state=mt.window().tolist(), random_seed=1234)
     setup = L.setup_of(ck)
+    assert setup.rng_seed == 1234
     assert setup.electrons[:2] == [0, 1] and len(setup.electrons) == int(0.05 * 256)
     # the seeded sites are the engine's draws from the restored state
     again = W.MT19937(window=ck.random_state)
@@ -309,3 +313,82 @@ def test_setup_of_checkpoint_resumes_engine():
     assert setup.electrons == expect
     kw = L.simulation_kwargs(ck, setup)
     assert np.array_equal(kw["rng_window"], again.window()) and kw["rng_consumed"] == again.consumed
+
+
+def reference_random_state(seed, blocks):
+    """[RandomState] text in the reference's layout, built without this package: the
+    seed (Random::operator<<, rand.cpp:359-364), then the twister's text state, which
+    the C++ standard fixes as the last 624 raw words, oldest first.  numpy's legacy
+    MT19937 regenerates its 624-word key in place like Boost, so right after it has
+    handed out a whole number of blocks (pos == 624) its key is exactly those words."""
+    rs = np.random.RandomState(seed)
+    rs.randint(0, 2 ** 32, size=624 * blocks, dtype=np.uint64)
+    name, key, pos = rs.get_state()[:3]
+    assert name == "MT19937" and pos == 624
+    text = f"[RandomState]\n{seed} " + " ".join(str(int(w)) for w in key) + "\n"
+    return text, key.astype(np.uint32), rs
+
+
+@pytest.mark.parametrize("seed,blocks", [(7, 0), (2024, 1), (4294967295, 3)])
+def test_reference_random_state_resumes_the_stream(seed, blocks):
+    """A 625-integer [RandomState] written the reference's way parses into the seed
+    and the engine window, and the engine resumed from it continues numpy's stream;
+    the checkpoint written back carries the same text."""
+    text, key, rs = reference_random_state(seed, blocks)
+    kv = L.KeyValues()
+    for line in ("grid.x = 16", "grid.y = 8", "hole.percentage = 0.05", "seed.charges = 1.0",
+                 "random.seed = 99", "e.source.l.rate = 0.0", "h.source.l.rate = 0.9", "h.drain.r.rate = 0.9"):
+        kv.parse_line(line)
+    ck = L.parse_checkpoint(text + "[Parameters]\n" + "".join(f"{ln}\n" for ln in
+                            ("grid.x = 16", "grid.y = 8", "random.seed = 99")))
+    assert ck.random_seed == seed and np.array_equal(np.asarray(ck.random_state, np.uint32), key)
+    mt = W.MT19937(window=ck.random_state)
+    expect = rs.randint(0, 2 ** 32, size=1500, dtype=np.uint64)
+    assert [mt() for _ in range(1500)] == [int(v) for v in expect]
+    # written back: the same section, seed first
+    ck2 = L.Checkpoint(parameters=kv, random_state=ck.random_state, random_seed=ck.random_seed)
+    out = L.format_checkpoint(ck2)
+    assert text.rstrip("\n") + "\n[Parameters]" in out
+    # a set-up from the file keeps the file's seed, not random.seed
+    ck3 = L.Checkpoint(parameters=kv, random_state=ck.random_state, random_seed=ck.random_seed)
+    assert L.setup_of(ck3).rng_seed == seed
+
+
+def test_checkpoint_of_simulation_writes_the_seed():
+    """Without a [RandomState] the engine is seeded from random.seed, and m_seed, the
+    word written before the twister, is random.seed cast to 32 bits (rand.cpp:36-47)."""
+    kv = L.KeyValues()
+    for line in ("grid.x = 16", "grid.y = 8", "hole.percentage = 0.05", "seed.charges = 1.0",
+                 "random.seed = 4294967301", "e.source.l.rate = 0.0", "h.source.l.rate = 0.9",
+                 "h.drain.r.rate = 0.9"):
+        kv.parse_line(line)
+    setup = L.setup_of(L.Checkpoint(parameters=kv))
+    assert setup.rng_seed == 5
+    state = dict(e_site=np.zeros(0, np.int32), h_site=np.asarray(setup.holes, np.int32),
+                 flux_attempts=np.zeros(10, np.uint64), flux_successes=np.zeros(10, np.uint64),
+                 rng_window=setup.rng_window, current_step=0)
+    text = L.format_checkpoint(L.checkpoint_of(state, kv, setup))
+    words = text.split("[RandomState]\n")[1].split("\n")[0].split()
+    assert len(words) == 625 and words[0] == "5"
+    assert [int(w) for w in words[1:]] == [int(w) for w in setup.rng_window]
+
+
+def test_gaussian_trap_depths_box_muller():
+    """gaussian.stdev > 0: after the drawn traps are placed, each gets trap.potential plus
+    one normal deviate, in list order (potential.cpp:211-220).  Each deviate is a fresh
+    Boost (1.44-1.55) normal_distribution: two uniform_01 draws r1, r2 (u32 * 2^-32) and
+    sqrt(-2 ln(1 - r2)) cos(2 pi r1) sigma.  Hand-derived from numpy's MT19937 words: the
+    engine position after the placement is known from a run without the deviation."""
+    kw = dict(grid_x=10, grid_y=10, grid_z=1, electron_percentage=0.02, seed_charges=1.0,
+              trap_percentage=0.08, seed_percentage=1.0, trap_potential=0.25, random_seed=31)
+    _, flat = params_pair(**kw)
+    _, gauss = params_pair(**{**kw, "gaussian_stdev": 0.07})
+    a, b = W.initialize(flat), W.initialize(gauss)
+    assert a.traps == b.traps and len(a.traps) == 8
+    words = np.random.RandomState(31).randint(0, 2 ** 32, size=a.rng_consumed + 16, dtype=np.uint64)
+    u = words[a.rng_consumed:].astype(np.float64) / 2.0 ** 32
+    expect = [0.25 + np.sqrt(-2.0 * np.log(1.0 - u[2 * i + 1])) * np.cos(2.0 * np.pi * u[2 * i]) * 0.07
+              for i in range(8)]
+    assert np.allclose(b.trap_potentials, expect, rtol=0, atol=1e-15)
+    assert b.rng_consumed == a.rng_consumed + 16
+    assert not np.allclose(b.trap_potentials, 0.25)

@@ -96,7 +96,8 @@ def test_morphology_drives_world_setup():
     ck = M.to_checkpoint(traps, (16, 12, 6))
     ck.parameters["random.seed"] = 3
     back = L.parse_checkpoint(L.format_checkpoint(L.Checkpoint(
-        parameters=ck.parameters, traps=ck.traps, random_state=W.MT19937(3).window().tolist())))
+        parameters=ck.parameters, traps=ck.traps, random_state=W.MT19937(3).window().tolist(),
+        random_seed=3)))
     setup = L.setup_of(back)
     assert setup.traps == traps.tolist()
     pot = setup.potential
