@@ -92,24 +92,91 @@ def transistor_workload(seed=2024):
 # ---------------------------------------------------------------------------
 # helpers
 # ---------------------------------------------------------------------------
-def in_cutoff_terms(dims, cutoff, sites, fsites, charges):
-    """Exact count of (target, charge) pairs with 0 < r^2 < c^2 over the 2N targets."""
+WARM_S = 2.0   # seconds of untimed steps before every timed window (shader clocks settle)
+
+
+def site_xyz(dims, sites):
+    """Site ids -> (x, y, z) by the reference's decode (cubicgrid.cpp:132-165), drain
+    slots V+k included (they decode past the lattice, as kernel.cl's staging does)."""
     X, Y, Z = dims
+    s = np.asarray(sites, dtype=np.int64)
+    z = s // (X * Y)
+    return np.stack([s % X, s // X - z * Y, z], 1).astype(np.float64)
 
-    def dec(s):
-        s = s.astype(np.int64)
-        z = s // (X * Y)
-        return np.stack([s % X, s // X - z * Y, z], 1).astype(np.int32)
 
-    c = dec(charges)
+def in_cutoff_terms(dims, cutoff, targets, charges):
+    """Exact count of (target, charge) pairs with 0 < r^2 < c^2 (integer r^2: r^2 <= c^2 - 1,
+    so a pair-count radius of sqrt(c^2 - 1/2) and a same-site radius of 1/2 are exact)."""
+    from scipy.spatial import cKDTree
+    if len(targets) == 0 or len(charges) == 0:
+        return 0
+    t, c = cKDTree(site_xyz(dims, targets)), cKDTree(site_xyz(dims, charges))
+    return int(t.count_neighbors(c, np.sqrt(cutoff * cutoff - 0.5))) - int(t.count_neighbors(c, 0.5))
+
+
+def map_in_cutoff_terms(dims, cutoff, sites):
+    """Exact count of (lattice site, charge) pairs with 0 < r^2 < c^2 for charges at distinct
+    sites: per charge, the lattice rows (dy, dz) inside the sphere, clipped to the box."""
+    X, Y, Z = dims
+    xyz = site_xyz(dims, sites).astype(np.int32)
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
     c2 = cutoff * cutoff
+    isqrt = np.array([int(np.floor(np.sqrt(v))) for v in range(c2)], dtype=np.int32)
+    isqrt = np.array([w - 1 if w * w > v else (w + 1 if (w + 1) ** 2 <= v else w) for v, w in enumerate(isqrt)],
+                     dtype=np.int32)
+    dy = np.arange(-(cutoff - 1), cutoff, dtype=np.int32)
     total = 0
-    for t in (dec(sites), dec(fsites)):
-        for a in range(0, len(t), 1024):
-            d = t[a:a + 1024, None, :] - c[None, :, :]
-            r2 = (d.astype(np.int32) ** 2).sum(-1)
-            total += int(((r2 > 0) & (r2 < c2)).sum())
-    return total
+    for dz in range(-(cutoff - 1), cutoff):
+        rem = c2 - 1 - dy * dy - dz * dz
+        keep = rem >= 0
+        zz = z + dz
+        zok = (zz >= 0) & (zz < Z)
+        if not keep.any() or not zok.any():
+            continue
+        d, w = dy[keep], isqrt[rem[keep]]
+        xs, ys = x[zok], y[zok]
+        yy = ys[None, :] + d[:, None]
+        lo = np.maximum(xs[None, :] - w[:, None], 0)
+        hi = np.minimum(xs[None, :] + w[:, None], X - 1)
+        total += int(np.where((yy >= 0) & (yy < Y), hi - lo + 1, 0).sum(dtype=np.int64))
+    return total - len(sites)   # the charge's own site (r = 0)
+
+
+def warm_until(step, sync, seconds, chunk):
+    """Untimed steps for a fixed wall time."""
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        step(chunk)
+        sync()
+        n += chunk
+    return n
+
+
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return dict(nproc=os.cpu_count(), affinity=affinity, model=model)
+
+
+def rel_errors(got, ref, floor=0.0):
+    """max |err|, RMS relative error ||err|| / ||ref||, max |err| / max(|ref|, floor)."""
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    err = np.abs(got - ref)
+    return dict(max_abs=float(err.max()) if err.size else 0.0,
+                rms_rel=float(np.linalg.norm(err) / max(np.linalg.norm(ref), 1e-300)),
+                max_rel=float((err / np.maximum(np.abs(ref), floor if floor > 0 else 1e-300)).max()) if err.size else 0.0)
 
 
 class HipTimer:
@@ -133,36 +200,61 @@ class HipTimer:
         return float(ms.value)
 
 
-def bench_map(reps=3):
-    """Full-lattice map, 256x256x64, 50,000 charges (+/-1 equal mix), cutoff 50 and none."""
-    import langmuir_amd as L
-    dims = (256, 256, 64)
-    V = 256 * 256 * 64
+MAP_DIMS = (256, 256, 64)
+MAP_SAMPLE = 4096
+
+
+def map_workload():
+    """256x256x64 lattice, 50,000 distinct uniform-random sites, 25,000 at +1 and 25,000 at -1."""
+    V = MAP_DIMS[0] * MAP_DIMS[1] * MAP_DIMS[2]
     rng = np.random.default_rng(7)
     n = 50000
     cs = rng.choice(V, n, replace=False).astype(np.int32)
     cq = np.where(np.arange(n) < n // 2, 1, -1).astype(np.int32)
-    out = {}
+    sample = np.sort(rng.choice(V, MAP_SAMPLE, replace=False)).astype(np.int32)
+    return cs, cq, sample
+
+
+def bench_map(reps=3):
+    """Full-lattice map, 256x256x64, 50,000 charges, cutoff 50 (culled and all pairs) and
+    no cutoff.  Roofline per variant: 11 FP32 flops per in-cutoff (site, charge) term over
+    the map's event-timed duration.  The maps' values at MAP_SAMPLE seeded sites are kept
+    for the FP64 comparison in the CPU leg."""
+    import langmuir_amd as L
+    dims = MAP_DIMS
+    V = dims[0] * dims[1] * dims[2]
+    cs, cq, sample = map_workload()
+    n = len(cs)
+    t0 = time.perf_counter()
+    terms50 = map_in_cutoff_terms(dims, 50, cs)
+    terms_all = V * n - n
+    log(f"map: in-cutoff terms {terms50:.4e} at cutoff 50, {terms_all:.4e} without ({time.perf_counter() - t0:.1f} s)")
+    out, samples = {}, {}
     ctx = L.Context(dims, cutoff=50)
     d_cs, d_cq = L.DeviceBuffer.from_array(cs), L.DeviceBuffer.from_array(cq)
     d_out = L.DeviceBuffer(4 * V)
     timer = HipTimer(ctx.stream)
-    # cutoff 50: the automatic (culled) map and the all-pairs form; no cutoff: all pairs
-    for name, flags in (("cutoff50", 0), ("cutoff50_all_pairs", L.LMG_MAP_PAIRS), ("nocutoff", L.LMG_NO_CUTOFF)):
-        ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags)
-        ctx.sync()
+    for name, flags, terms in (("cutoff50", 0, terms50), ("cutoff50_all_pairs", L.LMG_MAP_PAIRS, terms50),
+                               ("nocutoff", L.LMG_NO_CUTOFF, terms_all)):
+        warm_until(lambda k: [ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags) for _ in range(k)],
+                   ctx.sync, 0.5, 1)
         timer.start()
         for _ in range(reps):
             ctx.potential_map_dev(d_cs.ptr, d_cq.ptr, n, d_out.ptr, flags)
         ms = timer.stop_ms() / reps
-        pair_evals = V * n
+        v = d_out.download(np.float32, V)
+        samples[name] = v[sample].copy()
+        tflops = 11.0 * terms / (ms * 1e-3) / 1e12
         out[name] = dict(ms_per_map=round(ms, 4), maps_per_s=round(1e3 / ms, 3),
-                         pair_evals_per_s=pair_evals / (ms * 1e-3))
-        log(f"map {name}: {ms:.3f} ms/map, {pair_evals / (ms * 1e-3):.3e} pair-evals/s")
+                         pair_evals_per_s=V * n / (ms * 1e-3),
+                         roofline={"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                                   "per_launch": f"11 flops x {terms} in-cutoff terms"})
+        log(f"map {name}: {ms:.3f} ms/map, {V * n / (ms * 1e-3):.3e} pair-evals/s, {tflops:.2f} TFLOP/s")
     for a in (d_cs, d_cq, d_out):
         a.free()
     ctx.close()
-    return out
+    return out, dict(sample=sample, values=samples)
 
 
 BATCH_LATTICES = ((32, 32, 32), (48, 48, 24), (64, 64, 16), (64, 64, 32), (96, 96, 16), (128, 128, 16))
@@ -194,26 +286,54 @@ def batch_workload(n_rep=256, seed=4):
 
 
 def bench_batch(steps=200, warmup=20, n_rep=256):
+    """256 independent replicas stepped together.  value: replica-steps/s over `steps` graph
+    replays after a fixed-time warm-up; roofline of the replica-wide dE kernel from a
+    kernel-by-kernel run with events around it and exact in-cutoff term counts of every
+    replica's pending step."""
     import langmuir_amd as L
     reps = batch_workload(n_rep)
     t0 = time.perf_counter()
     batch = L.Batch(reps)
     log(f"batch of {n_rep} replicas created in {time.perf_counter() - t0:.1f} s")
+    # The replicas lose carriers through their drains as they run, so the clocks are warmed
+    # on a second copy of the batch and the timed one starts from the configured state.
+    heater = L.Batch(reps)
+    warm_until(heater.step, heater.sync, WARM_S, 20)
+    heater.close()
     batch.step(warmup)
     batch.sync()
-    ns = [len(batch.state(r)["h_site"]) for r in range(n_rep)]
     t0 = time.perf_counter()
     ms = batch.step_timed(steps)
     wall = time.perf_counter() - t0
+    ns, terms = [], 0
+    for r in range(n_rep):
+        st = batch.state(r)
+        fe, fh, _, _ = batch.delta_e(r)
+        p = reps[r]["params"]
+        sites = np.concatenate([st["e_site"], st["h_site"]])
+        ns.append(len(sites))
+        terms += in_cutoff_terms((p.grid_x, p.grid_y, p.grid_z), p.electrostatic_cutoff,
+                                 np.concatenate([sites, fe, fh]), sites)
+    n_ph = 50
+    de_ms, rest_ms, span_ms = batch.step_phases(n_ph)
+    de_us = de_ms * 1e3 / n_ph
+    tflops = 11.0 * terms / (de_us * 1e-6) / 1e12
     pair_evals = sum(2.0 * n * n for n in ns)
     out = dict(replicas=n_rep, steps=steps, ms_per_step=round(wall * 1e3 / steps, 4),
                event_ms_per_step=round(ms / steps, 4),
                replica_steps_per_s=round(n_rep * steps / wall, 1),
                pair_evals_per_s=pair_evals * steps / wall,
-               carriers_total=int(sum(ns)), graph_replay=True, autotuned=batch.tuning)
+               carriers_total=int(sum(ns)), graph_replay=True, autotuned=batch.tuning,
+               kernel_us={"de_kernel": round(de_us, 2), "rest_of_step": round(rest_ms * 1e3 / n_ph, 2),
+                          "event_span_per_step": round(span_ms * 1e3 / n_ph, 2),
+                          "source": f"{n_ph} steps kernel by kernel, HIP events around the dE kernel"},
+               roofline={"bound": "valu", "kernel": "de_kernel (all replicas, one launch)",
+                         "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                         "per_launch": f"11 flops x {terms} in-cutoff terms over all replicas"})
     log(f"batch: {out['ms_per_step']} ms/step for {n_rep} replicas -> "
-        f"{out['replica_steps_per_s']:.0f} replica-steps/s (dE chunks {out['autotuned']['charge_chunks']}, "
-        f"tuned dE kernel {out['autotuned']['de_kernel_us']:.0f} us)")
+        f"{out['replica_steps_per_s']:.0f} replica-steps/s (dE chunks {out['autotuned']['charge_chunks']}); "
+        f"dE kernel {de_us:.0f} us, {terms:.3e} terms, {tflops:.2f} TFLOP/s")
     batch.close()
     return out
 
@@ -306,7 +426,8 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
     shapes = surrogate_gemm_shapes(dims, sur.padded)
     alg = sum(2.0 * m * n * k * b for m, n, k, b in shapes)
     executed = alg * 3 - 2.0 * shapes[0][0] * shapes[0][1] * shapes[0][2]   # int8 stage: two products
-    tflops = executed / (ms_infer * 1e-3) / 1e12
+    tflops = alg / (ms_infer * 1e-3) / 1e12                  # algorithmic flops: one product per MAC
+    tflops_batch = alg * nb / (ms_batch * 1e-3) / 1e12       # the batch of gated maps (gates included)
     pz, py, kxn = sur.spectrum_shape
     free = (pz // 2 + 1) * (py // 2 + 1) * kxn
     log(f"surrogate: {n_snap} snapshots in {t_data:.1f} s, {iters} steps in {t_train:.1f} s (loss "
@@ -335,7 +456,11 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
         "mfma": {"bound": "mfma", "algorithmic_gflop_per_map": round(alg / 1e9, 4),
                  "executed_bf16_gflop_per_map": round(executed / 1e9, 4),
                  "achieved_tflops": round(tflops, 2), "peak_tflops": BF16_PEAK_TFLOPS,
-                 "frac": round(tflops / BF16_PEAK_TFLOPS, 5)},
+                 "frac": round(tflops / BF16_PEAK_TFLOPS, 5),
+                 "batch_achieved_tflops": round(tflops_batch, 2),
+                 "batch_frac": round(tflops_batch / BF16_PEAK_TFLOPS, 5),
+                 "note": "algorithmic flops (2 M N K per DFT GEMM) over one inference pass; the BF16 "
+                         "hi/lo split executes about 2.9x as many MFMA flops"},
     }
 
 
@@ -352,8 +477,8 @@ def measured_traffic(kernel, workload="solar"):
         return None
 
 
-def cpu_baseline(workload, budget_s=15.0):
-    """FP64 oracle (restated reference CPU path) on the same workload, all host cores."""
+def oracle_world(workload):
+    """The FP64 oracle's world of a bench workload (same lists, traps and engine seed)."""
     from oracle import pyoracle as O
     p, e, h, traps, pot = workload
     V = p.volume
@@ -365,28 +490,131 @@ def cpu_baseline(workload, budget_s=15.0):
               e_source_l_rate=p.e_source_l_rate, h_source_l_rate=p.h_source_l_rate,
               h_drain_r_rate=p.h_drain_r_rate, random_seed=p.random_seed,
               trap_percentage=(len(traps) + 0.5) / V if len(traps) else 0.0, trap_potential=0.5)
-    w = O.World(O.OrcParams.make(**kw), electrons=e, holes=h, traps=traps,
-                trap_potentials=np.full(len(traps), 0.5))
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    return O.World(O.OrcParams.make(**kw), electrons=e, holes=h, traps=traps,
+                   trap_potentials=np.full(len(traps), 0.5))
+
+
+def cpu_baseline(workload, budget_s=5.0, runs=3):
+    """The FP64 oracle (restated reference CPU path, OpenMP over carriers as QtConcurrent::map
+    spreads them) on the same workload from the same initial state, every core in the
+    affinity mask; median of `runs` samples of about budget_s seconds each."""
+    info = cpu_info()
+    threads = max(1, info["affinity"])
+    p, e, h, _, _ = workload
+    w = oracle_world(workload)
     t0 = time.perf_counter()
     w.step(1, threads)
     per = time.perf_counter() - t0
-    steps = int(max(2, min(2000, budget_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    w.step(steps, threads)
-    dt = time.perf_counter() - t0
+    steps = int(max(2, min(5000, budget_s / max(per, 1e-6))))
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        w.step(steps, threads)
+        rates.append(steps / (time.perf_counter() - t0))
     n = len(e) + len(h)
-    return dict(value=round(steps / dt, 4), unit="KMC steps/s", cores=threads, kind="port",
-                sample=f"{steps} steps of the same workload from the same initial state "
-                       f"(N={n} carriers), FP64 oracle, OpenMP over carriers, {dt:.1f} s",
-                pair_evals_per_s=2.0 * n * n * steps / dt)
+    med = float(np.median(rates))
+    return dict(value=round(med, 4), unit="KMC steps/s", cores=threads, kind="port",
+                sample=f"median of {runs} runs of {steps} steps each, continuing from the workload's initial "
+                       f"state (N={n} carriers at the start), FP64 oracle, OpenMP over carriers on {threads} "
+                       f"threads",
+                runs=[round(r, 4) for r in rates], nproc=info["nproc"], cpu_model=info["model"],
+                pair_evals_per_s=2.0 * n * n * med)
+
+
+def de_vs_fp64(workload, threads):
+    """dE of every carrier's first proposal on the device against the FP64 oracle from the
+    same state and engine (proposals must agree exactly)."""
+    import langmuir_amd as L
+    p, e, h, _, pot = workload
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot)
+    fe, fh, de_e, de_h = sim.delta_e()
+    sim.ctx.close()
+    w = oracle_world(workload)
+    w.propose()
+    w.coulomb(threads)
+    ce, ch = w.carriers(0), w.carriers(1)
+    assert np.array_equal(fe, ce["fsite"]) and np.array_equal(fh, ch["fsite"]), "proposals differ"
+    got, ref = np.concatenate([de_e, de_h]), np.concatenate([ce["de"], ch["de"]])
+    r = rel_errors(got, ref)
+    tol = 1e-5 * np.abs(ref) + 2e-7
+    r.update(carriers=int(len(ref)), within_tolerance=bool(np.all(np.abs(got - ref) <= tol)),
+             tolerance="1e-5 |dE| + 2e-7 eV")
+    return r
+
+
+def maps_vs_fp64(samples, threads):
+    """The device maps at the sampled sites against the FP64 oracle's pair sums there."""
+    from oracle import pyoracle as O
+    cs, cq, sample = map_workload()
+    out = {}
+    # no cutoff: any cutoff beyond the lattice diagonal (sqrt(255^2 + 255^2 + 63^2) < 366)
+    for name, cutoff in (("cutoff50", 50), ("nocutoff", 400)):
+        ref = O.pair_potential(MAP_DIMS, cutoff, cs, cq, sample)
+        r = rel_errors(samples["values"][name], ref)
+        r["sites"] = int(len(sample))
+        out[name] = r
+    return out
 
 
 # ---------------------------------------------------------------------------
+def step_leg(workload, steps, warmup, de_path, breakdown_steps):
+    """One simulation: `warmup` steps plus WARM_S seconds of untimed steps, then `steps`
+    timed graph replays between device synchronisations; then a kernel-by-kernel run of
+    `breakdown_steps` steps with HIP events for the phase and pair-sum-kernel times, with
+    exact in-cutoff term counts of the states entering and leaving it."""
+    import langmuir_amd as L
+    p, e, h, traps, pot = workload
+    dims = (p.grid_x, p.grid_y, p.grid_z)
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=de_path)
+    sim.step(warmup)
+    sim.ctx.sync()
+    warmed = warm_until(sim.step, sim.ctx.sync, WARM_S, 200)
+    st = sim.state()
+    n_start = len(st["e_site"]) + len(st["h_site"])
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.ctx.sync()
+    wall = time.perf_counter() - t0
+    assert sim.device_error() == 0
+    st = sim.state()
+    n_end = len(st["e_site"]) + len(st["h_site"])
+
+    def pending_terms():
+        s = sim.state()
+        fe, fh, _, _ = sim.delta_e()
+        sites = np.concatenate([s["e_site"], s["h_site"]])
+        return in_cutoff_terms(dims, p.electrostatic_cutoff, np.concatenate([sites, fe, fh]), sites)
+
+    fe0, fh0, _, _ = sim.delta_e()    # the breakdown starts from this state
+    s0 = sim.state()
+    de_ms, commit_ms, span_ms, pair_ms = sim.step_timed(breakdown_steps)
+    assert sim.device_error() == 0
+    sites0 = np.concatenate([s0["e_site"], s0["h_site"]])
+    terms = 0.5 * (in_cutoff_terms(dims, p.electrostatic_cutoff, np.concatenate([sites0, fe0, fh0]), sites0)
+                   + pending_terms())
+    kname = "cull_de_kernel" if sim.de_path == "culled" else "de_kernel"
+    pair_us = pair_ms * 1e3 / breakdown_steps
+    tflops = 11.0 * terms / (pair_us * 1e-6) / 1e12
+    n_avg = 0.5 * (n_start + n_end)
+    out = dict(steps_per_s=steps / wall, ms_per_step=wall * 1e3 / steps, warm_steps=int(warmup + warmed),
+               carriers_start=int(n_start), carriers_end=int(n_end), de_path=sim.de_path, de_shape=sim.tuning,
+               pair_evals_per_s=2.0 * n_avg * n_avg * steps / wall,
+               kernel_us={"de_phase": round(de_ms * 1e3 / breakdown_steps, 3), kname: round(pair_us, 3),
+                          "commit_phase": round(commit_ms * 1e3 / breakdown_steps, 3),
+                          "event_span_per_step": round(span_ms * 1e3 / breakdown_steps, 3),
+                          "source": f"{breakdown_steps} steps right after the timed window, kernel by kernel, "
+                                    "HIP events between the phases"},
+               roofline={"bound": "valu", "kernel": kname, "achieved": round(tflops, 4),
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                         "traffic": measured_traffic(kname, "solar" if p.is_solarcell else "transistor"),
+                         "traffic_source": "profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                           "scaled by a 256 MiB copy probe)",
+                         "per_launch": f"11 flops x {int(terms)} in-cutoff terms (mean of the states entering "
+                                       f"and leaving the breakdown run)"})
+    sim.ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,7 +623,9 @@ def main():
     ap.add_argument("--workload", choices=("solar", "transistor"), default="solar")
     ap.add_argument("--de-path", choices=("auto", "pairs", "culled"), default="auto",
                     help="dE path of the stepped simulation (auto: culled when it pays)")
+    ap.add_argument("--breakdown-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-transistor", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -406,7 +636,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (calibrates FETCH_SIZE/WRITE_SIZE "
                          "counters against a transfer of known size)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=5.0, help="seconds per CPU baseline sample")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
@@ -423,58 +653,24 @@ def main():
         b.free()
         log(f"copy probe: {args.copy_probe} MiB device to device")
     workload = solar_workload() if args.workload == "solar" else transistor_workload()
-    p, e, h, traps, pot = workload
-    log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(e)} e + {len(h)} h, "
+    p = workload[0]
+    log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(workload[1])} e + {len(workload[2])} h, "
         f"cutoff {p.electrostatic_cutoff}")
-    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=args.de_path)
-    sim.step(args.warmup)
-    sim.ctx.sync()
-    n_carriers = sum(len(v) for v in (sim.state()["e_site"], sim.state()["h_site"]))
-
-    # timed region: the steps as lmg_sim_step runs them (one captured step replayed as a
-    # HIP graph), bracketed by device synchronisation
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    sim.ctx.sync()
-    wall = time.perf_counter() - t0
-    assert sim.device_error() == 0
-    st2 = sim.state()
-    n_end = len(st2["e_site"]) + len(st2["h_site"])
-    steps_per_s = args.steps / wall
-    ms_per_step = wall * 1e3 / args.steps
-
-    # kernel breakdown and roofline: the same number of steps launched kernel by kernel with
-    # HIP events on the context stream before the dE phase, before the pair-sum kernel and
-    # after it, and after the commit phase; exact in-cutoff term count of the state entering it
-    st = sim.state()
-    fe, fh, _, _ = sim.delta_e()
-    sites = np.concatenate([st["e_site"], st["h_site"]])
-    fsites = np.concatenate([fe, fh])
-    terms = in_cutoff_terms((p.grid_x, p.grid_y, p.grid_z), p.electrostatic_cutoff, sites, fsites,
-                            sites)
-    log(f"N={n_carriers} at the timed region; in-cutoff terms per step {terms:.4e} (breakdown run)")
-    path = sim.de_path
-    kname = "cull_de_kernel" if path == "culled" else "de_kernel"
-    de_ms, commit_ms, span_ms, pair_ms = sim.step_timed(args.steps)
-    assert sim.device_error() == 0
-    de_avg = de_ms / args.steps
-    pair_avg = pair_ms / args.steps
-    flops = 11.0 * terms
-    achieved = flops / (pair_avg * 1e-3) / 1e12
-    n_avg = 0.5 * (n_carriers + n_end)
-    pair_evals_per_s = 2.0 * n_avg * n_avg * steps_per_s
-    log(f"{steps_per_s:.1f} steps/s ({ms_per_step * 1e3:.1f} us/step, graph replay); breakdown run: dE phase "
-        f"{de_avg * 1e3:.1f} us ({kname} {pair_avg * 1e3:.1f} us), commit {commit_ms / args.steps * 1e3:.1f} us, "
-        f"{span_ms / args.steps * 1e3:.1f} us per step with the events; {achieved:.2f} TFLOP/s")
+    main_leg = step_leg(workload, args.steps, args.warmup, args.de_path, max(args.breakdown_steps, 1))
+    k = main_leg["kernel_us"]
+    kname = main_leg["roofline"]["kernel"]
+    log(f"{main_leg['steps_per_s']:.1f} steps/s ({main_leg['ms_per_step'] * 1e3:.1f} us/step, graph replay, after "
+        f"{main_leg['warm_steps']} warm-up steps); breakdown: dE phase {k['de_phase']:.1f} us ({kname} "
+        f"{k[kname]:.1f} us), commit {k['commit_phase']:.1f} us; {main_leg['roofline']['achieved']:.2f} TFLOP/s")
 
     res = {
         "metric": METRIC,
-        "value": round(steps_per_s, 3),
+        "value": round(main_leg["steps_per_s"], 3),
         "unit": "KMC steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 6),
+        "ms_per_step": round(main_leg["ms_per_step"], 6),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -485,36 +681,50 @@ def main():
                          "with 0.5 eV traps, generation 0.524288/step, cutoff 30"
                          if args.workload == "solar" else
                          "transistor 256x256x1, 1310 holes, 1 V bias, cutoff 50"),
-            "carriers_start": int(n_carriers), "carriers_end": int(n_end),
-            "de_path": path,
-            "de_shape": sim.tuning,
+            "carriers_start": main_leg["carriers_start"], "carriers_end": main_leg["carriers_end"],
+            "de_path": main_leg["de_path"], "de_shape": main_leg["de_shape"],
+            "warm_steps_before_timing": main_leg["warm_steps"],
             "parallelism": "single GPU",
         },
-        "pair_evals_per_s": pair_evals_per_s,
-        "kernel_us": {"de_phase": round(de_avg * 1e3, 3),
-                      kname: round(pair_avg * 1e3, 3),
-                      "commit_phase": round(commit_ms / args.steps * 1e3, 3),
-                      "event_span_per_step": round(span_ms / args.steps * 1e3, 3),
-                      "source": "separate run of the same number of steps, kernel by kernel, HIP events "
-                                "between the phases"},
-        "roofline": {"bound": "valu", "kernel": kname, "achieved": round(achieved, 4),
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": measured_traffic(kname, args.workload),
-                     "traffic_source": "profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                       "scaled by a 256 MiB copy probe)",
-                     "per_launch": f"11 flops x {terms} in-cutoff terms"},
+        "pair_evals_per_s": main_leg["pair_evals_per_s"],
+        "pair_evals_note": "2 N^2 per step, the reference's all-pairs count; the culled path evaluates fewer",
+        "kernel_us": main_leg["kernel_us"],
+        "roofline": main_leg["roofline"],
     }
+    if not args.no_transistor and args.workload == "solar":
+        tr = step_leg(transistor_workload(), 3000, 200, "auto", 1000)
+        tr["steps_per_s"] = round(tr["steps_per_s"], 2)
+        res["transistor"] = tr
+        log(f"transistor: {tr['steps_per_s']:.1f} steps/s, {tr['roofline']['kernel']} "
+            f"{tr['kernel_us'][tr['roofline']['kernel']]:.1f} us, {tr['roofline']['achieved']:.2f} TFLOP/s")
+    map_samples = None
     if not args.no_map:
-        res["map"] = bench_map()
+        res["map"], map_samples = bench_map()
     if not args.no_batch:
         res["batch"] = bench_batch()
     if not args.no_surrogate and args.workload == "solar":
         res["surrogate"] = bench_surrogate(workload, n_snap=max(16, args.surrogate_snapshots),
                                            iters=args.surrogate_iters)
     if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
         cb = cpu_baseline(workload, args.cpu_budget)
         res["cpu_baseline"] = cb
-        res["speedup_vs_cpu"] = round(steps_per_s / cb["value"], 2)
+        res["speedup_vs_cpu"] = round(main_leg["steps_per_s"] / cb["value"], 2)
+        log(f"cpu baseline: {cb['value']:.3f} steps/s on {cb['cores']} threads ({cb['cpu_model']}), "
+            f"runs {cb['runs']}")
+        fp64 = {"dE": de_vs_fp64(workload, cb["cores"])}
+        if "transistor" in res:
+            tcb = cpu_baseline(transistor_workload(), max(1.0, args.cpu_budget / 2))
+            res["transistor"]["cpu_baseline"] = tcb
+            res["transistor"]["speedup_vs_cpu"] = round(res["transistor"]["steps_per_s"] / tcb["value"], 2)
+            fp64["dE_transistor"] = de_vs_fp64(transistor_workload(), cb["cores"])
+        if map_samples is not None:
+            fp64["map"] = maps_vs_fp64(map_samples, cb["cores"])
+        res["error_vs_fp64"] = fp64
+        log(f"FP64 comparison: dE rms rel {fp64['dE']['rms_rel']:.2e}, max abs {fp64['dE']['max_abs']:.2e} eV"
+            + ("" if map_samples is None else
+               f"; map cutoff50 rms rel {fp64['map']['cutoff50']['rms_rel']:.2e}, nocutoff "
+               f"{fp64['map']['nocutoff']['rms_rel']:.2e}") + f" ({time.perf_counter() - t0:.0f} s for the CPU leg)")
     print(json.dumps(res))
 
 

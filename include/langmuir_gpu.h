@@ -239,8 +239,17 @@ lmg_status lmg_batch_set_graphs(lmg_batch* batch, int32_t on);
 lmg_status lmg_batch_get_tuning(const lmg_batch* batch, int32_t* chunks, double* kernel_us);
 /* n steps timed with HIP events on the batch stream; *ms = event span */
 lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
+/* n steps launched kernel by kernel with HIP events around each replica-wide dE
+ * kernel: kernel_ms[0] = summed dE-kernel time, kernel_ms[1] = summed time of the
+ * rest of the step, kernel_ms[2] = first-to-last event span.  Same results as
+ * lmg_batch_step. */
+lmg_status lmg_batch_step_phases(lmg_batch* batch, int32_t n_steps, double* kernel_ms);
 lmg_status lmg_batch_sync(lmg_batch* batch);
 lmg_status lmg_batch_store(lmg_batch* batch, int32_t replica, lmg_sim_state* state);
+/* lmg_sim_delta_e of one replica: its pending proposals and their dE (waits for
+ * the batch stream first; consumes no draws). */
+lmg_status lmg_batch_delta_e(lmg_batch* batch, int32_t replica, int32_t* fsite_e, int32_t* fsite_h,
+                             double* de_e, double* de_h);
 
 /* ---------------------------------------------------------------------------
  * Learned surrogate of the potential map.  No counterpart in the reference

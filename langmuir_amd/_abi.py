@@ -123,6 +123,8 @@ def lib():
     L.lmg_batch_size.restype = I32
     L.lmg_batch_step.argtypes = [VP, I32]
     L.lmg_batch_step_timed.argtypes = [VP, I32, PF64]
+    L.lmg_batch_step_phases.argtypes = [VP, I32, PF64]
+    L.lmg_batch_delta_e.argtypes = [VP, I32, VP, VP, VP, VP]
     L.lmg_batch_sync.argtypes = [VP]
     L.lmg_batch_store.argtypes = [VP, I32, C.POINTER(LmgSimState)]
     L.lmg_get_params.argtypes = [VP, C.POINTER(LmgParams)]
@@ -157,7 +159,7 @@ EXPORTED = (
     "lmg_sim_get_de_path", "lmg_sim_delta_e",
     "lmg_sim_device_error", "lmg_sim_step_timed",
     "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
-    "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_sync", "lmg_batch_store",
+    "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_step_phases", "lmg_batch_delta_e", "lmg_batch_sync", "lmg_batch_store",
     "lmg_get_params",
     "lmg_surrogate_create", "lmg_surrogate_destroy", "lmg_surrogate_last_error", "lmg_surrogate_shape",
     "lmg_surrogate_get_weights", "lmg_surrogate_set_weights", "lmg_surrogate_occupancy_dev",

@@ -1087,14 +1087,21 @@ lmg_status bfail(lmg_batch* b, lmg_status code, const std::string& msg) {
             return bfail((b), LMG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-lmg_status batch_launch_step(lmg_batch* b) {
+// the dE kernel of every replica (pair sums, decision flags, random-stream refill)
+void batch_launch_de(lmg_batch* b) {
     const dim3 blk(kSimBlock);
     const int R = int(b->reps.size());
     const dim3 g_de(b->max_nblk + 1, b->coulomb ? b->max_S : 1, R);
-    const dim3 g_grid(b->max_nblk, 1, R), g_one(1, 1, R);
     if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_sims);
     else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_sims);
     else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_sims);
+}
+
+// decisions, recombination, ticks, injection and the next proposals of every replica
+lmg_status batch_launch_commit(lmg_batch* b) {
+    const dim3 blk(kSimBlock);
+    const int R = int(b->reps.size());
+    const dim3 g_grid(b->max_nblk, 1, R), g_one(1, 1, R);
     hipLaunchKernelGGL(decide_kernel, g_grid, blk, 0, b->stream, b->d_sims, 0);
     if (b->any_recomb) hipLaunchKernelGGL(recomb_kernel, g_one, dim3(kOneBlock), 0, b->stream, b->d_sims);
     hipLaunchKernelGGL(tick_kernel, g_grid, blk, 0, b->stream, b->d_sims);
@@ -1106,6 +1113,11 @@ lmg_status batch_launch_step(lmg_batch* b) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LMG_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
     return LMG_OK;
+}
+
+lmg_status batch_launch_step(lmg_batch* b) {
+    batch_launch_de(b);
+    return batch_launch_commit(b);
 }
 
 // n steps of every replica: graph replays of one captured step, or kernel by kernel
@@ -1284,6 +1296,41 @@ lmg_status lmg_batch_step_timed(lmg_batch* b, int32_t n, double* kernel_ms) {
     return LMG_OK;
 }
 
+lmg_status lmg_batch_step_phases(lmg_batch* b, int32_t n, double* kernel_ms) {
+    if (!b || !kernel_ms || n < 1) return bfail(b, LMG_ERR_ARG, "invalid lmg_batch_step_phases arguments");
+    BHIPCHK(b, hipSetDevice(b->dev));
+    const size_t need = 2 * size_t(n) + 1;
+    while (b->events.size() < need) {
+        hipEvent_t e;
+        BHIPCHK(b, hipEventCreate(&e));
+        b->events.push_back(e);
+    }
+    // per step: [2i] dE kernel starts, [2i + 1] dE kernel ends (commit starts), [2i + 2] commit ends
+    BHIPCHK(b, hipEventRecord(b->events[0], b->stream));
+    for (int i = 0; i < n; ++i) {
+        batch_launch_de(b);
+        BHIPCHK(b, hipEventRecord(b->events[2 * i + 1], b->stream));
+        const lmg_status st = batch_launch_commit(b);
+        if (st != LMG_OK) return st;
+        BHIPCHK(b, hipEventRecord(b->events[2 * i + 2], b->stream));
+    }
+    BHIPCHK(b, hipEventSynchronize(b->events[2 * n]));
+    double de = 0.0, cm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        float a = 0.f, c = 0.f;
+        BHIPCHK(b, hipEventElapsedTime(&a, b->events[2 * i], b->events[2 * i + 1]));
+        BHIPCHK(b, hipEventElapsedTime(&c, b->events[2 * i + 1], b->events[2 * i + 2]));
+        de += a;
+        cm += c;
+    }
+    float span = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&span, b->events[0], b->events[2 * n]));
+    kernel_ms[0] = de;
+    kernel_ms[1] = cm;
+    kernel_ms[2] = span;
+    return LMG_OK;
+}
+
 lmg_status lmg_batch_sync(lmg_batch* b) {
     if (!b) return LMG_ERR_ARG;
     BHIPCHK(b, hipSetDevice(b->dev));
@@ -1297,6 +1344,16 @@ lmg_status lmg_batch_store(lmg_batch* b, int32_t replica, lmg_sim_state* state) 
     BHIPCHK(b, hipSetDevice(b->dev));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     lmg_status st = lmg_sim_store(b->reps[replica], state);
+    if (st != LMG_OK) return bfail(b, st, lmg_last_error(b->reps[replica]));
+    return LMG_OK;
+}
+
+lmg_status lmg_batch_delta_e(lmg_batch* b, int32_t replica, int32_t* fsite_e, int32_t* fsite_h, double* de_e,
+                             double* de_h) {
+    if (!b || replica < 0 || replica >= int(b->reps.size())) return bfail(b, LMG_ERR_ARG, "invalid replica index");
+    BHIPCHK(b, hipSetDevice(b->dev));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    lmg_status st = lmg_sim_delta_e(b->reps[replica], fsite_e, fsite_h, de_e, de_h);
     if (st != LMG_OK) return bfail(b, st, lmg_last_error(b->reps[replica]));
     return LMG_OK;
 }

@@ -539,6 +539,22 @@ class Batch:
         self._chk(_abi.lib().lmg_batch_step_timed(self._h, int(n), C.byref(ms)))
         return float(ms.value)
 
+    def delta_e(self, replica: int):
+        """(fsite_e, fsite_h, dE_e, dE_h) of one replica's pending proposals."""
+        s = self.state(replica)
+        ne, nh = len(s["e_site"]), len(s["h_site"])
+        fe, fh = np.zeros(max(ne, 1), np.int32), np.zeros(max(nh, 1), np.int32)
+        de, dh = np.zeros(max(ne, 1)), np.zeros(max(nh, 1))
+        self._chk(_abi.lib().lmg_batch_delta_e(self._h, int(replica), _p(fe), _p(fh), _p(de), _p(dh)))
+        return fe[:ne], fh[:nh], de[:ne], dh[:nh]
+
+    def step_phases(self, n: int):
+        """n steps kernel by kernel with HIP events around the replica-wide dE kernel;
+        returns (dE kernel ms, rest of the step ms, event span ms), summed over the steps."""
+        out = (C.c_double * 3)()
+        self._chk(_abi.lib().lmg_batch_step_phases(self._h, int(n), out))
+        return float(out[0]), float(out[1]), float(out[2])
+
     def sync(self):
         self._chk(_abi.lib().lmg_batch_sync(self._h))
 
