@@ -97,6 +97,19 @@ lmg_status lmg_pair_potential_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
                                   const int32_t* d_target_site, int32_t n_targets,
                                   int32_t flags, double* d_out);
 
+/* lmg_pair_potential_dev that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its working
+ * memory (target and charge staging, per-chunk partials) is one fixed array per
+ * GPU, about 7.6 MiB, held until the process ends; it takes at most 65,536
+ * targets and 262,144 charges (LMG_ERR_CAPACITY beyond).  Results are the same
+ * bits as lmg_pair_potential_dev.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (any context, any stream) race. */
+lmg_status lmg_pair_potential_noalloc_dev(lmg_ctx* ctx, const int32_t* d_charge_site,
+                                          const int32_t* d_charge_q, int32_t n_charges,
+                                          const int32_t* d_target_site, int32_t n_targets,
+                                          int32_t flags, double* d_out);
+
 /* ---------------------------------------------------------------------------
  * Full-lattice potential map: the coulomb1/gauss1 kernels with
  * launchCoulombKernel1 (kernel.cl:56-220, openclhelper.cpp:220-370).  Writes

@@ -98,6 +98,7 @@ def lib():
     L.lmg_stream.restype = VP
     L.lmg_pair_potential.argtypes = [VP, VP, VP, I32, VP, I32, I32, VP]
     L.lmg_pair_potential_dev.argtypes = [VP, VP, VP, I32, VP, I32, I32, VP]
+    L.lmg_pair_potential_noalloc_dev.argtypes = [VP, VP, VP, I32, VP, I32, I32, VP]
     L.lmg_potential_map.argtypes = [VP, VP, VP, I32, I32, VP]
     L.lmg_potential_map_dev.argtypes = [VP, VP, VP, I32, I32, VP]
     L.lmg_sim_load.argtypes = [VP, C.POINTER(LmgSimConfig), C.POINTER(LmgSimState)]
@@ -152,7 +153,7 @@ def lib():
 
 EXPORTED = (
     "lmg_create", "lmg_destroy", "lmg_last_error", "lmg_device_count", "lmg_sync", "lmg_stream",
-    "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_potential_map", "lmg_potential_map_dev",
+    "lmg_pair_potential", "lmg_pair_potential_dev", "lmg_pair_potential_noalloc_dev", "lmg_potential_map", "lmg_potential_map_dev",
     "lmg_sim_load", "lmg_sim_store", "lmg_sim_step", "lmg_sim_set_sync", "lmg_sim_set_de_path",
     "lmg_sim_set_graphs", "lmg_batch_set_graphs", "lmg_sim_get_tuning", "lmg_batch_get_tuning",
     "lmg_sim_set_de_chunks",

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,7 +29,26 @@
 
 using namespace lmg;
 
+// Working memory of lmg_pair_potential_noalloc_dev, for callers that cannot allocate:
+// the float4 staging of the targets and charges and pair_kernel's per-chunk partials,
+// sized for at most kNoAllocTargets targets and kNoAllocCharges charges.  split_count
+// keeps S * nt <= (4096 + ceil(nt / 64) - 1) * 64, which bounds the partials.  Each GPU
+// holds one copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (g_scratch).
+constexpr int kNoAllocTargets = 65536;
+constexpr int kNoAllocCharges = 262144;
+constexpr size_t kNoAllocPartials = size_t(4096 + kNoAllocTargets / 64) * 64;
+constexpr size_t kScratchTgt = 0;
+constexpr size_t kScratchChg = kScratchTgt + size_t(kNoAllocTargets) * sizeof(float4);
+constexpr size_t kScratchPart = kScratchChg + size_t(kNoAllocCharges) * sizeof(float4);
+constexpr size_t kScratchBytes = kScratchPart + kNoAllocPartials * sizeof(float2);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};   // per device: the address of `scratch` there
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,6 +93,7 @@ struct lmg_ctx {
     SimCtl* ctl = nullptr;
     std::vector<void*> sim_allocs;
     std::vector<hipEvent_t> events;
+    void* scratch = nullptr;   // this device's copy of `scratch` (lmg_pair_potential_noalloc_dev)
 };
 
 namespace {
@@ -121,16 +142,11 @@ lmg_status check_sites(lmg_ctx* c, const int32_t* s, int n, int limit, const cha
     return LMG_OK;
 }
 
-lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, const int32_t* d_ts,
-                     int nt, int flags, double* d_out) {
-    lmg_status st;
-    if ((st = ensure(c, c->b_tgt, size_t(nt) * sizeof(float4))) != LMG_OK) return st;
-    if ((st = ensure(c, c->b_chg, size_t(std::max(nc, 1)) * sizeof(float4))) != LMG_OK) return st;
+// pair sums with the staging and partials in caller-provided device memory (tgt: nt,
+// chg: nc float4; part: split_count(nt, nc) * nt float2)
+lmg_status pair_launch(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, const int32_t* d_ts, int nt,
+                       int flags, double* d_out, float4* tgt, float4* chg, float2* part) {
     const int S = split_count(nt, nc);
-    if ((st = ensure(c, c->b_part, size_t(S) * nt * sizeof(float2))) != LMG_OK) return st;
-    float4* tgt = static_cast<float4*>(c->b_tgt.p);
-    float4* chg = static_cast<float4*>(c->b_chg.p);
-    float2* part = static_cast<float2*>(c->b_part.p);
     if (nc > 0) hipLaunchKernelGGL(to_float4_kernel, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d, d_cs, d_cq, nc, chg);
     hipLaunchKernelGGL(to_float4_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream, c->d, d_ts, nullptr, nt, tgt);
     const uint32_t c2m1 = cutoff_bits(c->p.cutoff, flags & LMG_NO_CUTOFF);
@@ -149,6 +165,33 @@ lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int n
     hipLaunchKernelGGL(pair_reduce_kernel, dim3((nt + kBlock / kRedLanes - 1) / (kBlock / kRedLanes)), dim3(kBlock), 0,
                        c->stream, part, nt, S, nt, scale, d_out);
     return launch_check(c, "pair_kernel");
+}
+
+lmg_status pair_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, const int32_t* d_ts,
+                     int nt, int flags, double* d_out) {
+    lmg_status st;
+    if ((st = ensure(c, c->b_tgt, size_t(nt) * sizeof(float4))) != LMG_OK) return st;
+    if ((st = ensure(c, c->b_chg, size_t(std::max(nc, 1)) * sizeof(float4))) != LMG_OK) return st;
+    const int S = split_count(nt, nc);
+    if ((st = ensure(c, c->b_part, size_t(S) * nt * sizeof(float2))) != LMG_OK) return st;
+    return pair_launch(c, d_cs, d_cq, nc, d_ts, nt, flags, d_out, static_cast<float4*>(c->b_tgt.p),
+                       static_cast<float4*>(c->b_chg.p), static_cast<float2*>(c->b_part.p));
+}
+
+// the address of `scratch` on the context's device, looked up once per device with
+// that device current
+lmg_status scratch_setup(lmg_ctx* c) {
+    if (c->dev >= kMaxDevices) return fail(c, LMG_ERR_ARG, "device ordinal above the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[c->dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, LMG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[c->dev] = p;
+    }
+    c->scratch = g_scratch[c->dev];
+    return LMG_OK;
 }
 
 // the culled map pays when the cutoff box of a 64-site tile covers well under
@@ -461,6 +504,12 @@ lmg_status lmg_create(const lmg_params* params, lmg_ctx** out) {
         delete c;
         return LMG_ERR_HIP;
     }
+    if (scratch_setup(c) != LMG_OK) {
+        g_create_error = c->err;
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return LMG_ERR_HIP;
+    }
     *out = c;
     return LMG_OK;
 }
@@ -505,6 +554,29 @@ lmg_status lmg_pair_potential_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t
     lmg_status st;
     if ((st = use_device(c)) != LMG_OK) return st;
     return pair_core(c, d_cs, d_cq, nc, d_ts, nt, flags, d_out);
+}
+
+lmg_status lmg_pair_potential_noalloc_dev(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int32_t nc,
+                                          const int32_t* d_ts, int32_t nt, int32_t flags, double* d_out) {
+    if (!c) return LMG_ERR_ARG;
+    if (nc < 0 || nt < 0 || (nc > 0 && (!d_cs || !d_cq)) || (nt > 0 && (!d_ts || !d_out)))
+        return fail(c, LMG_ERR_ARG, "invalid pair_potential arguments");
+    if (flags & ~(LMG_GAUSS | LMG_IMAGE_LEFT | LMG_NO_CUTOFF)) return fail(c, LMG_ERR_ARG, "unsupported flags");
+    if ((flags & LMG_GAUSS) && !(c->p.gaussian_sigma > 0))
+        return fail(c, LMG_ERR_ARG, "LMG_GAUSS needs gaussian_sigma > 0");
+    if (nt > kNoAllocTargets || nc > kNoAllocCharges)
+        return fail(c, LMG_ERR_CAPACITY, "lmg_pair_potential_noalloc_dev takes at most " +
+                                             std::to_string(kNoAllocTargets) + " targets and " +
+                                             std::to_string(kNoAllocCharges) + " charges");
+    if (nt == 0) return LMG_OK;
+    if (size_t(split_count(nt, nc)) * size_t(nt) > kNoAllocPartials)
+        return fail(c, LMG_ERR_CAPACITY, "partials exceed the scratch array");
+    if (!c->scratch) return fail(c, LMG_ERR_STATE, "no scratch address for this device");
+    lmg_status st;
+    if ((st = use_device(c)) != LMG_OK) return st;
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    return pair_launch(c, d_cs, d_cq, nc, d_ts, nt, flags, d_out, reinterpret_cast<float4*>(base + kScratchTgt),
+                       reinterpret_cast<float4*>(base + kScratchChg), reinterpret_cast<float2*>(base + kScratchPart));
 }
 
 lmg_status lmg_pair_potential(lmg_ctx* c, const int32_t* cs, const int32_t* cq, int32_t nc,
