@@ -87,11 +87,11 @@ def test_proposals_and_delta_e(gpu, oracle, name, de_path):
         assert np.all(err <= de_tolerance(ref)), f"max dE err {err.max():.3e}"
 
 
-def run_parity(w, sim, steps, beta):
+def run_parity(w, sim, steps, beta, threads=1):
     mismatched = 0
     for t in range(steps):
         w.propose()
-        w.coulomb()
+        w.coulomb(threads)
         ne, nh = w.info()["n_e"], w.info()["n_h"]
         de_e, de_h = w.carriers(0)["de"], w.carriers(1)["de"]
         w.finish()

@@ -392,3 +392,24 @@ def test_gaussian_trap_depths_box_muller():
     assert np.allclose(b.trap_potentials, expect, rtol=0, atol=1e-15)
     assert b.rng_consumed == a.rng_consumed + 16
     assert not np.allclose(b.trap_potentials, 0.25)
+
+
+@pytest.mark.parametrize("X", [48, 96, 33, 100, 128])
+def test_background_potential_rounds_like_the_reference(oracle, X):
+    """setPotentialLinear + setPotentialGate (potential.cpp:27-72): v = m (i + 0.5) + b with
+    m = (VR - VL) / X, then + slope.z (k + 0.5), each a product and a sum rounded
+    separately (no fused multiply-add).  The host's map and the oracle's are those bits."""
+    _, sp = params_pair(grid_x=X, grid_y=3, grid_z=5, electron_percentage=0.0, hole_percentage=0.05,
+                        seed_charges=1.0, voltage_left=1.0, voltage_right=-0.3, slope_z=0.1,
+                        e_source_l_rate=0.0, h_source_l_rate=0.9, h_drain_r_rate=0.9, random_seed=3)
+    m = (-0.3 - 1.0) / float(X)
+    expect = np.zeros((5, 3, X))
+    for i in range(X):
+        for k in range(5):
+            expect[k, :, i] = (0.0 + (m * (i + 0.5) + 1.0)) + 0.1 * (k + 0.5)
+    from langmuir_amd.host import background_potential
+    assert np.array_equal(background_potential(sp), expect.ravel())
+    op, _ = params_pair(grid_x=X, grid_y=3, grid_z=5, electron_percentage=0.0, hole_percentage=0.05,
+                        seed_charges=1.0, voltage_left=1.0, voltage_right=-0.3, slope_z=0.1,
+                        e_source_l_rate=0.0, h_source_l_rate=0.9, h_drain_r_rate=0.9, random_seed=3)
+    assert np.array_equal(oracle.World(op).potential(), expect.ravel())
