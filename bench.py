@@ -557,7 +557,7 @@ def maps_vs_fp64(samples, threads):
 
 
 # ---------------------------------------------------------------------------
-def step_leg(workload, steps, warmup, de_path, breakdown_steps):
+def step_leg(workload, steps, warmup, de_path, breakdown_steps, graphs=True):
     """One simulation: `warmup` steps plus WARM_S seconds of untimed steps, then `steps`
     timed graph replays between device synchronisations; then a kernel-by-kernel run of
     `breakdown_steps` steps with HIP events for the phase and pair-sum-kernel times, with
@@ -566,6 +566,7 @@ def step_leg(workload, steps, warmup, de_path, breakdown_steps):
     p, e, h, traps, pot = workload
     dims = (p.grid_x, p.grid_y, p.grid_z)
     sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=de_path)
+    sim.set_graphs(graphs)
     sim.step(warmup)
     sim.ctx.sync()
     warmed = warm_until(sim.step, sim.ctx.sync, WARM_S, 200)
@@ -624,6 +625,9 @@ def main():
     ap.add_argument("--de-path", choices=("auto", "pairs", "culled"), default="auto",
                     help="dE path of the stepped simulation (auto: culled when it pays)")
     ap.add_argument("--breakdown-steps", type=int, default=1000)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the steps kernel by kernel instead of replaying a captured HIP graph "
+                         "(for profilers that do not follow graph launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-transistor", action="store_true")
     ap.add_argument("--no-map", action="store_true")
@@ -656,7 +660,8 @@ def main():
     p = workload[0]
     log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(workload[1])} e + {len(workload[2])} h, "
         f"cutoff {p.electrostatic_cutoff}")
-    main_leg = step_leg(workload, args.steps, args.warmup, args.de_path, max(args.breakdown_steps, 1))
+    main_leg = step_leg(workload, args.steps, args.warmup, args.de_path, max(args.breakdown_steps, 1),
+                        graphs=not args.eager)
     k = main_leg["kernel_us"]
     kname = main_leg["roofline"]["kernel"]
     log(f"{main_leg['steps_per_s']:.1f} steps/s ({main_leg['ms_per_step'] * 1e3:.1f} us/step, graph replay, after "
@@ -692,7 +697,7 @@ def main():
         "roofline": main_leg["roofline"],
     }
     if not args.no_transistor and args.workload == "solar":
-        tr = step_leg(transistor_workload(), 3000, 200, "auto", 1000)
+        tr = step_leg(transistor_workload(), 3000, 200, "auto", 1000, graphs=not args.eager)
         tr["steps_per_s"] = round(tr["steps_per_s"], 2)
         res["transistor"] = tr
         log(f"transistor: {tr['steps_per_s']:.1f} steps/s, {tr['roofline']['kernel']} "

@@ -334,24 +334,8 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
     return launch_check(c, "de_kernel");
 }
 
-// proposals of the next step (inject_kernel makes them itself when some site
-// has fewer than two neighbours)
-lmg_status sim_launch_propose(lmg_ctx* c) {
-    const SimDev& D = c->sim;
-    // culled dE: propose_kernel also marks the carriers' bricks (cull_mark_kernel
-    // after serial proposals) and propose_fix_kernel, one workgroup of kSimBlock
-    // (cull_scan_kernel), turns the counts into offsets
-    if (D.uniform) {
-        hipLaunchKernelGGL(propose_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
-        hipLaunchKernelGGL(propose_fix_kernel, dim3(1), dim3(D.cull ? kSimBlock : 64), 0, c->stream, c->d_sim);
-    } else if (D.cull) {
-        hipLaunchKernelGGL(cull_mark_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, c->d_sim);
-        hipLaunchKernelGGL(cull_scan_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim);
-    }
-    return launch_check(c, "step kernels");
-}
-
-// decisions, recombination, completeTick/compaction, injection + proposals
+// decisions, recombination, completeTick/compaction, injection and the next
+// step's set-up (the proposals themselves are made by the next dE phase)
 lmg_status sim_launch_commit(lmg_ctx* c) {
     const SimDev& D = c->sim;
     const SimDev* P = c->d_sim;
@@ -360,7 +344,7 @@ lmg_status sim_launch_commit(lmg_ctx* c) {
         hipLaunchKernelGGL(recomb_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P);
     hipLaunchKernelGGL(tick_kernel, dim3(D.nblk), dim3(kSimBlock), 0, c->stream, P);
     hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, P, 0);
-    return sim_launch_propose(c);
+    return launch_check(c, "step kernels");
 }
 
 // ---------------------------------------------------------------------------
@@ -783,7 +767,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         (st = sim_alloc(c, D.h_at, d.V)) || (st = sim_alloc(c, D.code, cap)) ||
         (st = sim_alloc(c, D.removed, cap)) || (st = sim_alloc(c, D.flag, cap)) ||
         (st = sim_alloc(c, D.tgt1, cap)) || (st = sim_alloc(c, D.tgt2, cap)) ||
-        (st = sim_alloc(c, D.chg, cap + D.n_d_eff)) ||
+        (st = sim_alloc(c, D.chg, D.n_d_eff)) ||
         (st = sim_alloc(c, D.part, size_t(std::max(kMaxChunks, D.S_pairs)) * D.ld)) ||
         (st = sim_alloc(c, D.blk_draw, D.nblk)) || (st = sim_alloc(c, D.blk_keep, 2 * D.nblk)) ||
         (st = sim_alloc(c, D.de, cap)) || (st = sim_alloc(c, D.ring, ring)) || (st = sim_alloc(c, c->ctl, 1)) ||
@@ -937,7 +921,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     HIPCHK(c, hipMemcpy(c->ctl, &C, sizeof(C), hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
     hipLaunchKernelGGL(inject_kernel, dim3(1), dim3(kOneBlock), 0, c->stream, c->d_sim, 1);
-    if ((st = sim_launch_propose(c)) != LMG_OK) return st;
+    if ((st = launch_check(c, "inject_kernel(load)")) != LMG_OK) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((st = sim_tune_chunks(c)) != LMG_OK) return st;
     c->sim_loaded = true;
@@ -1136,7 +1120,7 @@ struct lmg_batch {
     int max_nblk = 0, max_S = 0;
     int tuned_S = 0;          // autotuned common charge-chunk count
     double tuned_us = -1.0;   // its dE-kernel time for all replicas
-    bool coulomb = false, gauss = false, any_uniform = false, any_recomb = false;
+    bool coulomb = false, gauss = false, any_recomb = false;
     bool use_graphs = true;
     hipGraphExec_t step_exec = nullptr;   // one step of every replica, captured at the first step
     std::vector<hipEvent_t> events;
@@ -1178,10 +1162,6 @@ lmg_status batch_launch_commit(lmg_batch* b) {
     if (b->any_recomb) hipLaunchKernelGGL(recomb_kernel, g_one, dim3(kOneBlock), 0, b->stream, b->d_sims);
     hipLaunchKernelGGL(tick_kernel, g_grid, blk, 0, b->stream, b->d_sims);
     hipLaunchKernelGGL(inject_kernel, g_one, dim3(kOneBlock), 0, b->stream, b->d_sims, 0);
-    if (b->any_uniform) {
-        hipLaunchKernelGGL(propose_kernel, g_grid, blk, 0, b->stream, b->d_sims);
-        hipLaunchKernelGGL(propose_fix_kernel, g_one, dim3(64), 0, b->stream, b->d_sims);
-    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, LMG_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
     return LMG_OK;
@@ -1278,7 +1258,6 @@ lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_c
         SimDev& D = b->reps[r]->sim;
         D.cull = 0;
         b->max_nblk = std::max(b->max_nblk, D.nblk);
-        b->any_uniform |= D.uniform != 0;
         b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
         max_charges = std::max(max_charges, D.cap_e + D.cap_h + D.n_d_eff);
     }

@@ -2,17 +2,21 @@
 // src/langmuirCore/simulation.cpp:29-111) as a short chain of kernels on one
 // stream, every carrier-parallel phase spread over the whole GPU:
 //
-//   de_kernel      (grid)   dE pair sums of every carrier's proposed hop, the
-//                           decision-draw flags, and (one extra workgroup) the
-//                           MT19937 refill for later steps
+//   de_kernel      (grid)   every carrier's proposal (one engine word each),
+//                           the dE pair sums of its hop, the decision-draw
+//                           flags, and (one extra workgroup) the MT19937
+//                           refill for later steps
 //   decide_kernel  (grid)   draw offsets (block prefix + in-block scan), dE
 //                           assembly, Metropolis / drain decisions, claims
 //   recomb_kernel  (1 WG)   RecombinationAgent sweep (solar cell, rate > 0)
 //   tick_kernel    (grid)   completeTick conflict rule and order-preserving
 //                           compaction of the lists
-//   inject_kernel  (1 WG)   injection (serial in the reference)
-//   propose_kernel (grid)   the next step's proposals, one engine word each
-//   propose_fix_kernel      serial redo after a (rare) Boost rejection
+//   inject_kernel  (1 WG)   injection (serial in the reference), then the
+//                           first proposal draw Boost's uniform_int rejects
+//                           and the serial proposals from there (rare)
+//
+// On the culled path cull_emit_kernel makes the proposals and cull_de_kernel
+// the pair sums (below).
 //
 // Offsets into the random stream are computed so that every draw uses exactly
 // the word the reference's serial loop would use.
@@ -36,7 +40,8 @@ struct SimCtl {
     int n_e_next, n_h_next;       // sizes after compaction (tick_kernel)
     int counts[2];                // de_kernel: targets, charges
     int first_bad;
-    int pad;
+    int danger_n;                 // engine outputs in Boost's rejection zone found by mt_refill
+    unsigned long long danger[16];// ... the last 16 of them (output indices)
     unsigned long long rng_pos;   // engine outputs consumed before the pending decisions
     unsigned long long rng_step0; // ... before the pending step's proposals
     unsigned long long gen_pos;   // raw words x[0 .. gen_pos) generated
@@ -102,7 +107,7 @@ struct SimDev {
     int* blk_keep;  // per workgroup: kept electrons, kept holes
     float4* tgt1;
     float4* tgt2;
-    float4* chg;    // [charged defects | electrons | holes]
+    float4* chg;    // the charged defects (carriers are read from their site lists)
     long long* part;              // de_kernel's chunk partials, [S][ld], int64 fixed point
     double* de;
     uint32_t* ring;
@@ -183,6 +188,13 @@ __device__ __forceinline__ long long wg_prefix_total(long long s, long long* slo
 // MT19937 refill: x[m] = x[m-227] ^ twist(x[m-624], x[m-623]), so 227
 // consecutive words can be produced at once from the last 624.
 // ---------------------------------------------------------------------------
+// Boost's uniform_int(0, r) rejects a word w iff w >= (r + 1) * bucket(r), which is at
+// least 2^32 - 1 - r.  A neighbour list holds at most 34 entries (hopping.range 2 in
+// three dimensions, 32, plus two drain slots), so no proposal draw below this can be
+// rejected; mt_refill lists the engine outputs at or above it.
+constexpr uint32_t kRejectZone = 0xffffffffu - 33u;
+constexpr int kDangerSlots = 16;
+
 __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsigned long long target) {
     unsigned long long gen = D.ctl->gen_pos;
     if (gen >= target) return;
@@ -198,11 +210,71 @@ __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsi
                                        tail[(m - 227ull) & 1023ull]);
             tail[m & 1023ull] = v;
             D.ring[m & D.ring_mask] = v;
+            if (mt_temper(v) >= kRejectZone) {   // engine output m - 624 (about one in 10^8)
+                const int slot = atomicAdd(&D.ctl->danger_n, 1);
+                D.ctl->danger[slot % kDangerSlots] = m - 624ull;
+            }
         }
         gen += 227ull;
         __syncthreads();
     }
     if (threadIdx.x == 0) D.ctl->gen_pos = gen;
+}
+
+// ===========================================================================
+// The pending step's proposals (chooseFuture, chargeagent.cpp:73-78), made by
+// the first kernel of the step that needs them.  With one draw per carrier
+// (every site has two or more neighbours), carrier i (electrons, then holes)
+// draws engine output rng_step0 + i unless a Boost uniform_int rejection
+// happened before it; inject_kernel finds the first rejecting carrier
+// (first_bad) and makes the proposals from there on serially, into fsite.
+// ===========================================================================
+struct Pending {
+    int site, fs;
+    float4 p1, p2;   // (x, y, z, q) at the current and the proposed site
+};
+
+__device__ __forceinline__ Pending pending_proposal(const SimDev& D, int i, int ne, int cur,
+                                                    unsigned long long step0, int first_bad) {
+    const bool isE = i < ne;
+    const int li = isE ? i : i - ne;
+    const float q = isE ? -1.f : 1.f;
+    Pending P;
+    P.site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+    if (P.site < 0 || P.site >= D.d.V) {   // never written by the step kernels: a corrupt state
+        set_error(D, kErrSite);
+        P.site = 0;
+    }
+    int x, y, z, fx, fy, fz;
+    decode(D.d, P.site, x, y, z);
+    if (i < first_bad) {
+        const int dl = isE ? D.e_drain[0] : D.h_drain[0];
+        const int dr = isE ? D.e_drain[1] : D.h_drain[1];
+        const uint32_t w = rng_word(D, step0 + (unsigned long long)i);
+        if (D.hop == 1) {
+            const int n = nbr1_count(D.d, x, y, z, dl, dr);
+            const int k = min(int(w / uint_bucket(uint32_t(n - 1))), n - 1);
+            P.fs = nbr1_pick(D.d, x, y, z, dl, dr, k, fx, fy, fz);
+        } else {
+            const int n = nbr_count(D.d, D.hop, P.site, dl, dr);
+            const int k = min(int(w / uint_bucket(uint32_t(n - 1))), n - 1);
+            P.fs = nbr_at(D.d, D.hop, P.site, dl, dr, k);
+            decode(D.d, P.fs, fx, fy, fz);
+        }
+    } else {
+        P.fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+        decode(D.d, P.fs, fx, fy, fz);   // a drain slot V + k decodes past the lattice (kernel.cl:229-231)
+    }
+    P.p1 = make_float4(float(x), float(y), float(z), q);
+    P.p2 = make_float4(float(fx), float(fy), float(fz), q);
+    return P;
+}
+
+// charge j of the all-pairs list [charged defects | electrons | holes]
+__device__ __forceinline__ float4 charge_f4(const SimDev& D, int j, int nd, int ne, int cur) {
+    if (j < nd) return D.chg[j];
+    j -= nd;
+    return j < ne ? site_f4(D.d, D.e_site[cur][j], -1.f) : site_f4(D.d, D.h_site[cur][j - ne], 1.f);
 }
 
 // ===========================================================================
@@ -258,15 +330,21 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
             mt_refill(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
-    const int ne = D.ctl->n_e, nt = D.ctl->counts[0], nc = D.ctl->counts[1];
+    const SimCtl& C = *D.ctl;
+    const int ne = C.n_e, nt = C.counts[0], nc = C.counts[1], cur = C.cur, nd = D.n_d_eff;
     if (int(blockIdx.x) * kSimBlock >= nt) return;   // uniform per workgroup
     const int i = blockIdx.x * kSimBlock + threadIdx.x;
+    // the target's proposal (every charge chunk makes it; chunk 0 records it)
+    const int fb = C.first_bad;
+    Pending P;
+    P.p1 = P.p2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < nt) P = pending_proposal(D, i, ne, cur, C.rng_step0, fb);
     if (blockIdx.y == 0) {
         int draw = 0;
         if (i < nt) {
             const bool isE = i < ne;
-            const int fs = isE ? D.e_fsite[i] : D.h_fsite[i - ne];
-            const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
+            if (i < fb) (isE ? D.e_fsite : D.h_fsite)[isE ? i : i - ne] = P.fs;
+            const uint8_t tf = isE ? D.etype[P.fs] : D.htype[P.fs];
             draw = (tf == kEmpty || tf == kDrain);
             D.flag[i] = draw;
         }
@@ -278,8 +356,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
     const int S = D.S, nb32 = (nc + 31) / 32;
     const int j0 = 32 * int((long long)blockIdx.y * nb32 / S);
     const int j1 = min(nc, 32 * int((long long)(blockIdx.y + 1) * nb32 / S));
-    const float4 p1 = i < nt ? D.tgt1[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 p2 = i < nt ? D.tgt2[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 p1 = P.p1, p2 = P.p2;
     const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
     const float hh = hx * hx + hy * hy + hz * hz;
     const uint32_t c2m1 = D.c2m1;
@@ -288,7 +365,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
     for (int base = j0; base < j1; base += 256) {
         const int n = min(256, j1 - base);
         __syncthreads();
-        if (threadIdx.x < n) tile[threadIdx.x] = D.chg[base + threadIdx.x];
+        if (threadIdx.x < n) tile[threadIdx.x] = charge_f4(D, base + threadIdx.x, nd, ne, cur);
         __syncthreads();
         int k0 = 0;
         for (; k0 + 32 <= n; k0 += 32) {   // full blocks: unrolled, branch-free
@@ -320,13 +397,15 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 // Morton order over the long ones), so a group's current and proposed sites
 // fill a small box.
 //
-//   cull_mark (propose_kernel, or cull_mark_kernel after serial proposals):
-//     each carrier's bitmask bit, per-brick charge and target counts, and the
+//   cull_mark (tick_kernel for the kept carriers at their new sites,
+//     inject_kernel for the injected ones, or for all after a load): each
+//     carrier's bitmask bit, per-brick charge and target counts, and the
 //     carrier's slot among its brick's targets.
-//   propose_fix_kernel (or cull_scan_kernel): the counts become list offsets.
+//   inject_kernel: the counts become list offsets.
 //   cull_emit_kernel: one wave per brick writes its charge list; the other
-//     workgroups set the decision-draw flags and put the targets into their
-//     brick's slots; the last one writes cull_de_kernel's work items.
+//     workgroups make the proposals, set the decision-draw flags and put the
+//     targets into their brick's slots; the last one writes cull_de_kernel's
+//     work items.
 //   cull_de_kernel: the pair sums of every target over the lists of the bricks
 //     near its group (below), as int64 fixed-point totals.
 //   decide_kernel (committing pass) clears the bitmask words and brick
@@ -350,37 +429,39 @@ __device__ __forceinline__ int load_agent(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// carrier i (current site) into its brick: bitmask bit, charge count, target slot
-__device__ __forceinline__ void cull_mark(const SimDev& D, int i, bool isE, int site) {
+// a carrier at its current site into its brick: bitmask bit, charge count, and
+// its slot among the brick's targets, kept per list entry (electron li at
+// tpos[li], hole li at tpos[cap_e + li]: an injected electron does not move
+// the holes' entries)
+__device__ __forceinline__ void cull_mark(const SimDev& D, bool isE, int li, int site) {
     int x, y, z;
     decode(D.d, site, x, y, z);
     uint32_t bit;
     atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
     const int b = cull_brick(D, x, y, z);
     atomicAdd(&D.scnt[b], 1);
-    D.tpos[i] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
+    D.tpos[isE ? li : D.cap_e + li] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
 }
 
-// grid over carriers: cull_mark for every carrier (after inject_kernel made
-// the proposals serially; propose_kernel marks its carriers itself)
-__global__ __launch_bounds__(kSimBlock) void cull_mark_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
-    const int ne = D.ctl->n_e, nt = ne + D.ctl->n_h, cur = D.ctl->cur;
-    const int i = blockIdx.x * kSimBlock + threadIdx.x;
-    if (i >= nt) return;
-    const bool isE = i < ne;
-    cull_mark(D, i, isE, isE ? D.e_site[cur][i] : D.h_site[cur][i - ne]);
-}
-
-// Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the targets (bricks in bperm order; tst[brick])
-// by one workgroup of kThreads; thread t scans a contiguous range of bricks,
-// its first kPer counts loaded together.
-template <int kThreads>
-__device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
+// Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the
+// targets (bricks in bperm order; tst[brick]) by one workgroup of kThreads;
+// thread t scans a contiguous range of bricks, its first kPer counts loaded
+// together.  Each brick's counts are the marks of the brick counters plus the
+// carriers listed in inj_b (bricks of carriers not yet counted there).  kAgent
+// reads the counters at agent scope (needed when this workgroup marked them).
+template <int kThreads, bool kAgent>
+__device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, const int* inj_b, int inj_n) {
     constexpr int kPer = 8;
     const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
     const int b0 = min(nb, int(threadIdx.x) * per), b1 = min(nb, b0 + per);
-    auto scount = [&](int b) { return D.scnt[b] + D.dstart[b + 1] - D.dstart[b]; };
+    auto ld = [&](const int* p) { return kAgent ? load_agent(p) : *p; };
+    auto injc = [&](int b) {
+        int n = 0;
+        for (int k = 0; k < inj_n; ++k) n += inj_b[k] == b;
+        return n;
+    };
+    auto scount = [&](int b) { return ld(&D.scnt[b]) + injc(b) + D.dstart[b + 1] - D.dstart[b]; };
+    auto tcount = [&](int b) { return ld(&D.tcnt[b]) + injc(b); };
     // unconditional loads (clamped index) so that all are in flight together
     int bp[kPer], sc[kPer], tc[kPer];
     int srun = 0, trun = 0;
@@ -390,7 +471,7 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
     for (int u = 0; u < kPer; ++u) {
         const int b = min(b0 + u, nb - 1);
         sc[u] = scount(b);
-        tc[u] = D.tcnt[bp[u]];
+        tc[u] = tcount(bp[u]);
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -400,7 +481,7 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
     }
     for (int b = b0 + kPer; b < b1; ++b) {
         srun += scount(b);
-        trun += D.tcnt[D.bperm[b]];
+        trun += tcount(D.bperm[b]);
     }
     int soff, toff;
     int so = wg_scan<kThreads>(srun, wsum, soff);
@@ -420,16 +501,9 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum) {
         sst[b] = so;
         tst[q] = to;
         so += scount(b);
-        to += D.tcnt[q];
+        to += tcount(q);
     }
     if (threadIdx.x == 0) sst[nb] = soff;
-}
-
-// the offsets after cull_mark_kernel (propose_fix_kernel computes them after
-// propose_kernel)
-__global__ __launch_bounds__(kOneBlock) void cull_scan_kernel(const SimDev* __restrict__ Dp) {
-    __shared__ int wsum[kOneBlock / 64];
-    cull_offsets<kOneBlock>(*Dp, Dp->sstart, Dp->tstart, wsum);
 }
 
 // Work items of cull_de_kernel, one workgroup of kSimBlock: group g gets
@@ -475,19 +549,24 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
         cull_items(D, wsum);
         return;
     }
-    if (int(blockIdx.x) >= nbw) {   // targets: decision-draw flags, then into their brick's slots
-        const int ne = D.ctl->n_e, nt = D.ctl->counts[0];
+    if (int(blockIdx.x) >= nbw) {   // targets: proposals, decision-draw flags, their brick's slots
+        const SimCtl& C = *D.ctl;
+        const int ne = C.n_e, nt = C.counts[0];
         const int i = (blockIdx.x - nbw) * kSimBlock + threadIdx.x;
         int draw = 0;
         if (i < nt) {
+            const bool isE = i < ne;
+            const int li = isE ? i : i - ne;
+            const Pending P = pending_proposal(D, i, ne, C.cur, C.rng_step0, C.first_bad);
+            if (i < C.first_bad) (isE ? D.e_fsite : D.h_fsite)[li] = P.fs;
+            D.tgt1[i] = P.p1;
+            D.tgt2[i] = P.p2;
             D.isum[i] = 0;
             // decision draws: own-grid type Empty or Drain at the proposed site (chargeagent.cpp:95-157)
-            const bool isE = i < ne;
-            const int fs = isE ? D.e_fsite[i] : D.h_fsite[i - ne];
-            const uint8_t tf = isE ? D.etype[fs] : D.htype[fs];
+            const uint8_t tf = isE ? D.etype[P.fs] : D.htype[P.fs];
             draw = (tf == kEmpty || tf == kDrain);
             D.flag[i] = draw;
-            const int2 tp = D.tpos[i];
+            const int2 tp = D.tpos[isE ? li : D.cap_e + li];
             D.torder[tst[tp.x] + tp.y] = i;
         }
         const int cnt = __syncthreads_count(draw);
@@ -986,6 +1065,10 @@ struct Serial {
     __device__ bool choose_yes(double p) { return p > random(); }   // rand.cpp:117-124
 };
 
+// carriers one injection pass can add: the two face or exciton passes, and
+// balanceCharges' two loops of at most 1025 tries each
+constexpr int kInjMax = 2 * 1025 + 8;
+
 struct OneShared {
     int wsum[16];
     unsigned long long pos;
@@ -996,6 +1079,9 @@ struct OneShared {
     unsigned int att[10], suc[10];
     double red[16];
     uint32_t tail[1024];
+    // carriers injected this step on the culled path: brick, and tpos index
+    int inj_n;
+    int inj_b[kInjMax], inj_t[kInjMax];
 };
 
 __device__ double wg_sum_d(double v, double* red) {
@@ -1143,17 +1229,37 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
         D.e_site[nxt][w] = dest;
         D.e_life[nxt][w] = life;
         D.e_path[nxt][w] = path;
+        if (D.cull) cull_mark(D, true, w, dest);   // the next step's brick lists
     } else if (keep_h) {
         const int w = int(base_h) + oh;
         D.h_site[nxt][w] = dest;
         D.h_life[nxt][w] = life;
         D.h_path[nxt][w] = path;
+        if (D.cull) cull_mark(D, false, w, dest);
     }
 }
 
 // ===========================================================================
 // Injection (simulation.cpp:233-317, sourceagent.cpp:92-247) and proposals.
 // ===========================================================================
+// an injected carrier on the culled path: its bitmask bit now; its brick goes
+// into the list that inject_kernel's offsets add and then turns into counts and
+// a target slot (cull_mark's other half)
+__device__ void inject_mark(const SimDev& D, OneShared& sh, bool isE, int li, int site) {
+    int x, y, z;
+    decode(D.d, site, x, y, z);
+    uint32_t bit;
+    atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
+    const int k = sh.inj_n;
+    if (k >= kInjMax) {
+        set_error(D, kErrCapacity);
+        return;
+    }
+    sh.inj_b[k] = cull_brick(D, x, y, z);
+    sh.inj_t[k] = isE ? li : D.cap_e + li;
+    sh.inj_n = k + 1;
+}
+
 __device__ void append_carrier(const SimDev& D, OneShared& sh, bool isE, int site) {
     const int cur = sh.cur;
     if (isE) {
@@ -1161,12 +1267,14 @@ __device__ void append_carrier(const SimDev& D, OneShared& sh, bool isE, int sit
         if (n >= D.cap_e) { set_error(D, kErrCapacity); return; }
         D.e_site[cur][n] = site; D.e_life[cur][n] = 0; D.e_path[cur][n] = 0;
         D.etype[site] = kElectron;
+        if (D.cull) inject_mark(D, sh, true, n, site);
         sh.n_e = n + 1;
     } else {
         const int n = sh.n_h;
         if (n >= D.cap_h) { set_error(D, kErrCapacity); return; }
         D.h_site[cur][n] = site; D.h_life[cur][n] = 0; D.h_path[cur][n] = 0;
         D.htype[site] = kHole;
+        if (D.cull) inject_mark(D, sh, false, n, site);
         sh.n_h = n + 1;
     }
 }
@@ -1315,42 +1423,6 @@ __device__ int proposal_count(const SimDev& D, bool isE, int site) {
     return nbr_count(D.d, D.hop, site, dl, dr);
 }
 
-__device__ void stage_carrier(const SimDev& D, int nd, int i, bool isE, int site, int fs) {
-    const float q = isE ? -1.f : 1.f;
-    const float4 p1 = site_f4(D.d, site, q);
-    D.tgt1[i] = p1;
-    D.tgt2[i] = site_f4(D.d, fs, q);
-    D.chg[nd + i] = p1;
-}
-
-// chooseFuture for one carrier from engine word w (one decode per carrier);
-// returns false when Boost's uniform_int would reject w
-__device__ __forceinline__ bool propose_one(const SimDev& D, int nd, int i, bool isE, int li, int site,
-                                            uint32_t w) {
-    const int dl = isE ? D.e_drain[0] : D.h_drain[0];
-    const int dr = isE ? D.e_drain[1] : D.h_drain[1];
-    const float q = isE ? -1.f : 1.f;
-    int x, y, z, fx, fy, fz, fs, n;
-    decode(D.d, site, x, y, z);
-    if (D.hop == 1) n = nbr1_count(D.d, x, y, z, dl, dr);
-    else n = nbr_count(D.d, D.hop, site, dl, dr);
-    const uint32_t range = uint32_t(n - 1);
-    const uint32_t r = w / uint_bucket(range);
-    const int k = min(int(r), n - 1);
-    if (D.hop == 1) {
-        fs = nbr1_pick(D.d, x, y, z, dl, dr, k, fx, fy, fz);
-    } else {
-        fs = nbr_at(D.d, D.hop, site, dl, dr, k);
-        decode(D.d, fs, fx, fy, fz);
-    }
-    if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
-    const float4 p1 = make_float4(float(x), float(y), float(z), q);
-    D.tgt1[i] = p1;
-    D.tgt2[i] = make_float4(float(fx), float(fy), float(fz), q);
-    D.chg[nd + i] = p1;
-    return r <= range;
-}
-
 // Serial chooseFuture from carrier i0 on, starting at engine output pos
 // (thread 0 only): the rare path after a Boost rejection, and lattices where
 // some site has fewer than two neighbours (a one-entry list draws nothing).
@@ -1364,7 +1436,6 @@ __device__ unsigned long long propose_serial(const SimDev& D, int i0, int ne, in
         if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); continue; }
         const int k = R.integer(0, proposal_count(D, isE, site) - 1);
         const int fs = proposal_site(D, isE, site, k);
-        stage_carrier(D, D.n_d_eff, i, isE, site, fs);
         if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
     }
     return pos;
@@ -1372,8 +1443,13 @@ __device__ unsigned long long propose_serial(const SimDev& D, int i0, int ne, in
 
 // mode 0: injection for the pending step (after tick_kernel)
 // mode 1: refill the random stream after a state load
-// Leaves the new lists' sizes, the proposal start rng_step0 and counts in ctl;
-// with D.uniform == 0 it also makes the proposals (serially).
+// Then the next step's set-up: on the culled path the brick counters (mode 1
+// marks every carrier; in mode 0 tick_kernel and the injections marked them)
+// become list offsets; the proposal draws are checked for a Boost rejection
+// (first_bad), from which the proposals are made serially here -- all of them
+// when some site has fewer than two neighbours (one-entry lists draw nothing).
+// Leaves the new lists' sizes, the proposal start rng_step0, first_bad, the
+// decisions' start rng_pos and the counts in ctl.
 __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restrict__ Dp, int mode) {
     const SimDev& D = Dp[blockIdx.z];   // replica
     __shared__ OneShared sh;
@@ -1395,63 +1471,98 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
         sh.att[threadIdx.x] = 0;
         sh.suc[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0) {
+        sh.first_bad = 0x7fffffff;
+        sh.inj_n = 0;
+    }
     __syncthreads();
     if (mode == 1) mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
     else inject_all(D, sh);
     __syncthreads();
+    const int ne = sh.n_e, N = ne + sh.n_h, cur = sh.cur;
+    const unsigned long long step0 = sh.pos;
+    // Carrier i's proposal draw is output step0 + i; the first one Boost rejects.  Only
+    // outputs that mt_refill listed can be rejected: thread 0 checks those in range.
+    // Should the list have wrapped past one of them, every carrier is checked.
+    if (D.uniform && N > 0 && threadIdx.x == 0) {
+        const int nd = C.danger_n;
+        unsigned long long oldest = ~0ull;
+        for (int k = 0; k < min(nd, kDangerSlots); ++k) {
+            const unsigned long long p = C.danger[k];
+            oldest = p < oldest ? p : oldest;
+            if (p < step0 || p >= step0 + (unsigned long long)N) continue;
+            const int i = int(p - step0);
+            const bool isE = i < ne;
+            const int site = isE ? D.e_site[cur][i] : D.h_site[cur][i - ne];
+            if (site < 0 || site >= D.d.V) continue;   // reported by the proposal below
+            const uint32_t range = uint32_t(proposal_count(D, isE, site) - 1);
+            if (rng_word(D, p) / uint_bucket(range) > range) sh.first_bad = min(sh.first_bad, i);
+        }
+        sh.valid = nd > kDangerSlots && oldest > step0;   // the full check is needed
+    }
+    __syncthreads();
+    if (D.uniform && N > 0 && sh.valid) {
+        constexpr int kB = 8;      // carriers per thread per round, their loads in flight together
+        for (int i0 = threadIdx.x; i0 < N; i0 += kB * kOneBlock) {
+            int site[kB];
+            uint32_t w[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int i = min(i0 + u * kOneBlock, N - 1);
+                site[u] = i < ne ? D.e_site[cur][i] : D.h_site[cur][i - ne];
+                w[u] = rng_word(D, step0 + (unsigned long long)i);
+            }
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int i = i0 + u * kOneBlock;
+                if (i >= N) continue;
+                if (site[u] < 0 || site[u] >= D.d.V) {
+                    set_error(D, kErrSite);
+                    continue;
+                }
+                const uint32_t range = uint32_t(proposal_count(D, i < ne, site[u]) - 1);
+                if (w[u] / uint_bucket(range) > range) atomicMin(&sh.first_bad, i);
+            }
+        }
+    }
+    if (D.cull) {
+        if (mode == 1) {   // after a load: every carrier, then the offsets over counters marked here
+            for (int i = threadIdx.x; i < N; i += blockDim.x) {
+                const bool isE = i < ne;
+                const int li = isE ? i : i - ne;
+                cull_mark(D, isE, li, isE ? D.e_site[cur][li] : D.h_site[cur][li]);
+            }
+            __syncthreads();
+            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0);
+        } else {   // tick_kernel's marks plus this step's injected carriers
+            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n);
+            __syncthreads();
+            for (int k = threadIdx.x; k < sh.inj_n; k += blockDim.x) {
+                const int b = sh.inj_b[k];
+                atomicAdd(&D.scnt[b], 1);
+                D.tpos[sh.inj_t[k]] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
+            }
+        }
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const int N = sh.n_e + sh.n_h;
-        C.n_e = sh.n_e;
+        const int fb = D.uniform ? min(sh.first_bad, N) : 0;
+        if (step0 + (unsigned long long)N + 624ull > C.gen_pos) set_error(D, kErrRing);
+        C.n_e = ne;
         C.n_h = sh.n_h;
-        C.cur = sh.cur;
-        C.rng_step0 = sh.pos;
-        C.first_bad = 0x7fffffff;
+        C.cur = cur;
+        C.rng_step0 = step0;
+        C.first_bad = fb;
         C.counts[0] = N;
         C.counts[1] = D.n_d_eff + N;
         if (mode == 0) C.step += 1;
-        if (!D.uniform) C.rng_pos = propose_serial(D, 0, sh.n_e, N, sh.cur, sh.pos);
+        C.rng_pos = fb < N ? propose_serial(D, fb, ne, N, cur, step0 + (unsigned long long)fb)
+                           : step0 + (unsigned long long)N;
     }
     if (threadIdx.x < 10 && mode == 0) {
         C.att[threadIdx.x] += sh.att[threadIdx.x];
         C.suc[threadIdx.x] += sh.suc[threadIdx.x];
     }
-}
-
-// chooseFuture for every carrier when every site has >= 2 neighbours (drains
-// on both x faces): carrier i uses engine output rng_step0 + i unless a Boost
-// rejection happened before it; the first rejecting index is recorded.
-__global__ __launch_bounds__(kSimBlock) void propose_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = Dp[blockIdx.z];   // replica
-    if (!D.uniform) return;             // inject_kernel proposed serially
-    const SimCtl& C = *D.ctl;
-    const int ne = C.n_e, N = ne + C.n_h, cur = C.cur;
-    const int i = blockIdx.x * kSimBlock + threadIdx.x;
-    if (i >= N) return;
-    const bool isE = i < ne;
-    const int li = isE ? i : i - ne;
-    const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-    if (site < 0 || site >= D.d.V) { set_error(D, kErrSite); return; }
-    if (!propose_one(D, D.n_d_eff, i, isE, li, site, rng_word(D, C.rng_step0 + (unsigned long long)i)))
-        atomicMin(&D.ctl->first_bad, i);
-    if (D.cull) cull_mark(D, i, isE, site);
-}
-
-// After propose_kernel: redo serially from the first rejection (if any) and
-// set the engine position of the pending decisions.
-__global__ __launch_bounds__(kSimBlock) void propose_fix_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = Dp[blockIdx.z];   // replica
-    SimCtl& C = *D.ctl;
-    if (D.cull) {   // one workgroup of kSimBlock: the culled dE's offsets first
-        __shared__ int wsum[kSimBlock / 64];
-        cull_offsets<kSimBlock>(D, D.sstart, D.tstart, wsum);
-    }
-    if (threadIdx.x != 0 || !D.uniform) return;
-    const int ne = C.n_e, N = ne + C.n_h;
-    const unsigned long long base = C.rng_step0;
-    if (base + N + 624ull > C.gen_pos) set_error(D, kErrRing);
-    const int fb = C.first_bad;
-    C.rng_pos = fb < N ? propose_serial(D, fb, ne, N, C.cur, base + (unsigned long long)fb)
-                       : base + (unsigned long long)N;
 }
 
 }  // namespace lmg
