@@ -735,7 +735,9 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     D.nby = (d.Y + kBrick - 1) / kBrick;
     D.nbz = (d.Z + kBrick - 1) / kBrick;
     D.nbrick = D.nbx * D.nby * D.nbz;
-    if (D.coulomb && D.nbrick <= kMaxCullBricks) {
+    // (the culled kernel addresses list slots in 24 bits)
+    if (D.coulomb && D.nbrick <= kMaxCullBricks &&
+        size_t(cap) + size_t(D.n_d_eff) + 3 * size_t(D.nbrick) < (size_t(1) << 24)) {
         auto frac = [&](int n) { return std::min(1.0, double(16 + 2 * c->p.cutoff) / n); };
         const bool worth = frac(d.X) * frac(d.Y) * frac(d.Z) < 0.6 && double(cap) * (cap + D.n_d_eff) >= 8e6;
         D.cull = c->de_mode == 2 || (c->de_mode == 0 && worth);

@@ -290,14 +290,25 @@ __device__ __forceinline__ float4 charge_f4(const SimDev& D, int j, int nd, int 
 //   1/sqrt(r2) - 1/sqrt(r1) = (r1 - r2) (ab)^2 / (a + b),  a = r1^-1/2, b = r2^-1/2,
 // with r1 - r2 an exact integer.
 // ===========================================================================
+// The hop h = p2 - p1 of a target enters a term through
+//   |p2 - q|^2 = r1 + t,  t = 2 h.(p1 - q) + |h|^2 = K - 2 h.q,  K = 2 h.p1 + |h|^2,
+// all exact integers in FP32; HopTerm holds -2h and K for one target.
+struct HopTerm {
+    float ux, uy, uz, k;
+};
+__device__ __forceinline__ HopTerm hop_term(const float4& p1, const float4& p2) {
+    const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
+    const float k = 2.f * (hx * p1.x + hy * p1.y + hz * p1.z) + (hx * hx + hy * hy + hz * hz);
+    return HopTerm{-2.f * hx, -2.f * hy, -2.f * hz, k};
+}
+
 // [0<r2<c2] f(r2) - [0<r1<c2] f(r1) for one charge q, target pair (p1, p1+h)
 template <bool kGauss>
-__device__ __forceinline__ float de_term(const float4& p1, const float4& q, float hx, float hy, float hz,
-                                         float hh, uint32_t c2m1, float gk) {
+__device__ __forceinline__ float de_term(const float4& p1, const float4& q, const HopTerm& h, uint32_t c2m1,
+                                         float gk) {
     const float dx = p1.x - q.x, dy = p1.y - q.y, dz = p1.z - q.z;
     const float r1 = dx * dx + dy * dy + dz * dz;
-    // |p2 - q|^2 = r1 + t with t = 2 h.(p1 - q) + |h|^2, exact in integers
-    const float t = fmaf(2.f, hx * dx + hy * dy + hz * dz, hh);
+    const float t = fmaf(h.ux, q.x, fmaf(h.uy, q.y, fmaf(h.uz, q.z, h.k)));
     const float r2 = r1 + t;
     const bool in1 = cut_in(r1, c2m1), in2 = cut_in(r2, c2m1);
     const float a = rsqf(r1), b = rsqf(r2);
@@ -356,9 +367,8 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
     const int S = D.S, nb32 = (nc + 31) / 32;
     const int j0 = 32 * int((long long)blockIdx.y * nb32 / S);
     const int j1 = min(nc, 32 * int((long long)(blockIdx.y + 1) * nb32 / S));
-    const float4 p1 = P.p1, p2 = P.p2;
-    const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
-    const float hh = hx * hx + hy * hy + hz * hz;
+    const float4 p1 = P.p1;
+    const HopTerm ht = hop_term(P.p1, P.p2);
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
     long long acc64 = 0;
@@ -372,13 +382,13 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
             float acc = 0.f;
 #pragma unroll 8
             for (int k = k0; k < k0 + 32; ++k)
-                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
+                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
             acc64 += (long long)(acc * kFix);
         }
         if (k0 < n) {   // the list's last, partial block
             float acc = 0.f;
             for (int k = k0; k < n; ++k)
-                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], hx, hy, hz, hh, c2m1, gk), acc);
+                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
             acc64 += (long long)(acc * kFix);
         }
     }
@@ -704,8 +714,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const int bz0 = max(0, z0 - c + 1) >> 3, bz1 = min(D.d.Z - 1, z1 + c - 1) >> 3;
     const int nx = bx1 - bx0 + 1, ny = by1 - by0 + 1, ncand = nx * ny * (bz1 - bz0 + 1);
     const float inx = 1.f / float(nx), iny = 1.f / float(ny);
-    const float hx = p2.x - p1.x, hy = p2.y - p1.y, hz = p2.z - p1.z;
-    const float hh = hx * hx + hy * hy + hz * hz;
+    const HopTerm ht = hop_term(p1, p2);
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
     const int* __restrict__ sstart = D.sstart;
@@ -748,12 +757,12 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             float acc = 0.f;
             for (; t + 2 <= e; t += 2) {
                 const float4 q0 = lists[wv][t], q1 = lists[wv][t + 1];
-                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, hx, hy, hz, hh, c2m1, gk), acc);
-                acc = fmaf(q1.w, de_term<kGauss>(p1, q1, hx, hy, hz, hh, c2m1, gk), acc);
+                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, ht, c2m1, gk), acc);
+                acc = fmaf(q1.w, de_term<kGauss>(p1, q1, ht, c2m1, gk), acc);
             }
             if (t < e) {
                 const float4 q0 = lists[wv][t++];
-                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, hx, hy, hz, hh, c2m1, gk), acc);
+                acc = fmaf(q0.w, de_term<kGauss>(p1, q0, ht, c2m1, gk), acc);
             }
             acc64 += (long long)(acc * kFix);
         }
@@ -853,16 +862,40 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         }
         const int mincl = wave_incl_sum(mine);
         if (wave_total(mincl) <= kList) {   // the whole chunk in one gather
-            int o = mincl - mine;
+            // the chunk's units in lane and candidate order: (first slot << 7) | length
             int uo = wave_incl_sum(munits) - munits;
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) {
-                if (td[j] > 0) copy(k8[j] + fr[j], td[j], o, uo);
-                o += td[j];
-                uo += (td[j] + kUnit - 1) / kUnit;
-            }
-            fill = wave_total(mincl);
+            for (int j = 0; j < kPer; ++j)
+                for (int k = 0; k < td[j]; k += kUnit)
+                    uends[wv][uo++] = ((k8[j] + fr[j] + k) << 7) | min(kUnit, td[j] - k);
             nu = wave_total(wave_incl_sum(munits));
+            __builtin_amdgcn_wave_barrier();
+            // every lane copies one slot of a unit, two units at a time; the near
+            // charges of each unit are packed in order (ballot prefix)
+            const v4f* src = reinterpret_cast<const v4f*>(D.src);
+            v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]);
+            int o = 0;
+            const v4f far = {-1e4f, -1e4f, -1e4f, 0.f};
+            for (int u = 0; u < nu; u += 2) {
+                const int d0 = __builtin_amdgcn_readfirstlane(uends[wv][u]);
+                const int d1 = u + 1 < nu ? __builtin_amdgcn_readfirstlane(uends[wv][u + 1]) : 0;
+                const bool in0 = lane < (d0 & 127), in1 = lane < (d1 & 127);
+                const v4f a0 = in0 ? src[(d0 >> 7) + lane] : far;
+                const v4f a1 = in1 ? src[(d1 >> 7) + lane] : far;
+                const bool k0 = in0 && near(a0), k1 = in1 && near(a1);
+                const uint64_t m0 = __ballot(k0), m1 = __ballot(k1);
+                const int n0 = __popcll(m0);
+                if (k0) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m0 >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0))] = a0;
+                uends[wv][u] = (o << 16) | (o + n0);
+                o += n0;
+                if (u + 1 < nu) {
+                    const int n1 = __popcll(m1);
+                    if (k1) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m1 >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m1), 0))] = a1;
+                    uends[wv][u + 1] = (o << 16) | (o + n1);
+                    o += n1;
+                }
+            }
+            fill = o;
         } else {
 #pragma unroll
             for (int j = 0; j < kPer; ++j) segment(k8[j] + fr[j], td[j]);

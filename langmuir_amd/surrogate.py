@@ -11,6 +11,7 @@ whose judge is the exact pair sum, with the exact map as fallback.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from typing import Iterable, Optional, Sequence, Tuple
 
 import numpy as np
@@ -19,6 +20,40 @@ from . import _abi
 from ._abi import LmgError
 from .device import DeviceBuffer
 from .host import Context, Simulation
+
+
+def analytic_spectrum(dims: Sequence[int], padded: Sequence[int], cutoff: int, prefactor: float,
+                      sigma: float = 0.0, no_cutoff: bool = False) -> np.ndarray:
+    """The weights that make the surrogate the exact map, without training: the
+    half-spectrum (numpy's rfftn over z, y, x) of the map's kernel laid out
+    circularly on the padded grid.  Index k of a padded axis of length P stands for
+    the offset k (k < N) or k - P (k > P - N) of a lattice axis of length N; the
+    kernel there is prefactor * f(r) with f = 1/r, times erf(r / (sqrt(2) sigma))
+    when sigma > 0, over 0 < r^2 < cutoff^2 (every r > 0 with no_cutoff): the sum
+    kernel.cl's coulomb1/gauss1 evaluate (kernel.cl:56-220)."""
+    X, Y, Z = (int(n) for n in dims)
+    px, py, pz = (int(n) for n in padded)
+
+    def offsets(n, p):
+        k = np.arange(p)
+        d = np.where(k < n, k, k - p)
+        return d, (k < n) | (k > p - n)
+
+    ox, mx = offsets(X, px)
+    oy, my = offsets(Y, py)
+    oz, mz = offsets(Z, pz)
+    r2 = (oz[:, None, None] ** 2 + oy[None, :, None] ** 2 + ox[None, None, :] ** 2).astype(np.int64)
+    keep = mz[:, None, None] & my[None, :, None] & mx[None, None, :] & (r2 > 0)
+    if not no_cutoff:
+        keep &= r2 < int(cutoff) ** 2
+    r = np.sqrt(r2[keep].astype(np.float64))
+    f = 1.0 / r
+    if sigma > 0:
+        erf = np.frompyfunc(math.erf, 1, 1)
+        f = f * erf(r / (math.sqrt(2.0) * sigma)).astype(np.float64)
+    kern = np.zeros((pz, py, px))
+    kern[keep] = prefactor * f
+    return np.fft.rfftn(kern, axes=(0, 1, 2))
 
 
 class Surrogate:
@@ -84,6 +119,13 @@ class Surrogate:
         flat = np.empty(2 * self.n_weights, np.float32)
         flat[0::2], flat[1::2] = w.real, w.imag
         self._chk(_abi.lib().lmg_surrogate_set_weights(self._h, flat.ctypes.data_as(C.c_void_p)))
+
+    def set_analytic_weights(self) -> None:
+        """Load the exact operator's spectrum (:func:`analytic_spectrum`) instead of
+        trained weights: the same pipeline, with nothing learned."""
+        p = self.ctx.params
+        self.set_weights(analytic_spectrum(self.dims, self.padded, p.cutoff, p.prefactor, p.gaussian_sigma,
+                                           bool(self.flags & _abi.LMG_NO_CUTOFF)))
 
     # -- device-pointer entry points ---------------------------------------------
     def occupancy_dev(self, site_ptr: int, q_ptr: int, n: int, occ_ptr: int) -> None:

@@ -112,3 +112,17 @@ def test_truncated_modes_cannot_meet_the_gate():
 def test_occupancy_sums_coincident_charges():
     occ = S.occupancy((4, 2, 1), [0, 0, 3, 5, 5, 5], [-1, 1, 1, -1, -1, 1])
     assert occ.tolist() == [0, 0, 0, 1, 0, -1, 0, 0]
+
+
+@pytest.mark.parametrize("dims,cutoff,sigma,no_cut", [((20, 12, 6), 5, 0.0, False), ((16, 16, 8), 30, 0.0, False),
+                                                      ((12, 10, 4), 3, 1.3, False), ((9, 7, 5), 0, 0.0, True)])
+def test_product_analytic_spectrum_is_the_ideal_operator(dims, cutoff, sigma, no_cut):
+    """langmuir_amd.surrogate.analytic_spectrum (the untrained, exact weights the product
+    can load) equals the oracle's ideal weights, which reproduce the FP64 map."""
+    from langmuir_amd.surrogate import analytic_spectrum
+    pref = 0.41141839736460467
+    padded = S.padded_shape(dims, cutoff, no_cut)
+    got = analytic_spectrum(dims, padded, cutoff, pref, sigma, no_cut)
+    ref = S.ideal_weights(dims, cutoff, pref, sigma, no_cut)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
