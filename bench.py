@@ -285,7 +285,7 @@ def batch_workload(n_rep=256, seed=4):
     return reps
 
 
-def bench_batch(steps=200, warmup=20, n_rep=256):
+def bench_batch(steps=200, warmup=20, n_rep=256, de_path="auto"):
     """256 independent replicas stepped together.  value: replica-steps/s over `steps` graph
     replays after a fixed-time warm-up; roofline of the replica-wide dE kernel from a
     kernel-by-kernel run with events around it and exact in-cutoff term counts of every
@@ -293,11 +293,11 @@ def bench_batch(steps=200, warmup=20, n_rep=256):
     import langmuir_amd as L
     reps = batch_workload(n_rep)
     t0 = time.perf_counter()
-    batch = L.Batch(reps)
+    batch = L.Batch(reps, de_path=de_path)
     log(f"batch of {n_rep} replicas created in {time.perf_counter() - t0:.1f} s")
     # The replicas lose carriers through their drains as they run, so the clocks are warmed
     # on a second copy of the batch and the timed one starts from the configured state.
-    heater = L.Batch(reps)
+    heater = L.Batch(reps, de_path=de_path)
     warm_until(heater.step, heater.sync, WARM_S, 20)
     heater.close()
     batch.step(warmup)
@@ -314,6 +314,7 @@ def bench_batch(steps=200, warmup=20, n_rep=256):
         ns.append(len(sites))
         terms += in_cutoff_terms((p.grid_x, p.grid_y, p.grid_z), p.electrostatic_cutoff,
                                  np.concatenate([sites, fe, fh]), sites)
+    paths = batch.de_paths
     n_ph = 50
     de_ms, rest_ms, span_ms = batch.step_phases(n_ph)
     de_us = de_ms * 1e3 / n_ph
@@ -324,16 +325,20 @@ def bench_batch(steps=200, warmup=20, n_rep=256):
                replica_steps_per_s=round(n_rep * steps / wall, 1),
                pair_evals_per_s=pair_evals * steps / wall,
                carriers_total=int(sum(ns)), graph_replay=True, autotuned=batch.tuning,
-               kernel_us={"de_kernel": round(de_us, 2), "rest_of_step": round(rest_ms * 1e3 / n_ph, 2),
+               culled_replicas=paths.count("culled"),
+               kernel_us={"de_phase": round(de_us, 2), "rest_of_step": round(rest_ms * 1e3 / n_ph, 2),
                           "event_span_per_step": round(span_ms * 1e3 / n_ph, 2),
-                          "source": f"{n_ph} steps kernel by kernel, HIP events around the dE kernel"},
-               roofline={"bound": "valu", "kernel": "de_kernel (all replicas, one launch)",
+                          "source": f"{n_ph} steps kernel by kernel, HIP events around the dE phase"},
+               roofline={"bound": "valu",
+                         "kernel": "dE phase of all replicas (culled replicas: cull_emit_kernel + cull_de_kernel, "
+                                   "one launch each; the rest: de_kernel, one launch)",
                          "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
                          "per_launch": f"11 flops x {terms} in-cutoff terms over all replicas"})
     log(f"batch: {out['ms_per_step']} ms/step for {n_rep} replicas -> "
-        f"{out['replica_steps_per_s']:.0f} replica-steps/s (dE chunks {out['autotuned']['charge_chunks']}); "
-        f"dE kernel {de_us:.0f} us, {terms:.3e} terms, {tflops:.2f} TFLOP/s")
+        f"{out['replica_steps_per_s']:.0f} replica-steps/s ({out['culled_replicas']} culled, "
+        f"{out['autotuned']['cull_workgroups']} culled workgroups, all-pairs chunks "
+        f"{out['autotuned']['charge_chunks']}); dE phase {de_us:.0f} us, {terms:.3e} terms, {tflops:.2f} TFLOP/s")
     batch.close()
     return out
 
@@ -647,6 +652,8 @@ def main():
     ap.add_argument("--no-transistor", action="store_true")
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-de-path", choices=("auto", "pairs", "culled"), default="auto",
+                    help="dE path of the batch's replicas (auto: the batch autotuner decides)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-iters", type=int, default=1200,
                     help="training steps of the surrogate (fewer for profiler runs)")
@@ -721,7 +728,7 @@ def main():
     if not args.no_map:
         res["map"], map_samples = bench_map()
     if not args.no_batch:
-        res["batch"] = bench_batch()
+        res["batch"] = bench_batch(de_path=args.batch_de_path)
     if not args.no_surrogate and args.workload == "solar":
         res["surrogate"] = bench_surrogate(workload, n_snap=max(16, args.surrogate_snapshots),
                                            iters=args.surrogate_iters)

@@ -189,11 +189,13 @@ lmg_status lmg_sim_set_sync(lmg_ctx* ctx, int32_t synchronous);
  * the first step after a load) when on (the default), or launches the kernels
  * one by one.  Same results either way.  No counterpart in the reference. */
 lmg_status lmg_sim_set_graphs(lmg_ctx* ctx, int32_t on);
-/* The ΔE kernel's shape after the load: on the all-pairs path the number of
- * charge chunks, picked by timing each candidate on the loaded state, and its
- * measured time in microseconds; on the culled path the number of work items
- * (sized from the occupancy query) and *kernel_us = -1.  The sums do not
- * depend on either.  No counterpart in the reference. */
+/* The ΔE kernel's shape after the load, picked by timing each candidate on the
+ * loaded state, and its measured time in microseconds: on the all-pairs path
+ * the number of charge chunks; on the culled path the number of work items (1
+ * to 3 device-resident waves of workgroups, four items each; the time covers
+ * the list building too).  A forced shape (lmg_sim_set_de_chunks) reports
+ * *kernel_us = -1.  The sums do not depend on either.  No counterpart in the
+ * reference. */
 lmg_status lmg_sim_get_tuning(const lmg_ctx* ctx, int32_t* shape, double* kernel_us);
 /* Fixes the ΔE kernel's shape for the following loads instead of choosing it
  * (0, the default): on the all-pairs path the charge-chunk count (1..128), on
@@ -241,15 +243,33 @@ typedef struct lmg_batch lmg_batch;
 
 lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
                             const lmg_sim_state* states, int32_t device_id, lmg_batch** out);
+/* lmg_batch_create with a choice of ΔE path: 0 automatic (the replicas whose
+ * cutoff box around a target group covers well under their lattice are
+ * candidates for the culled path; the autotuner times them culled against all
+ * of the batch on the all-pairs path and keeps the faster), 1 every replica on
+ * the all-pairs path, 2 every replica that can be culled (Coulomb on and the
+ * lattice within the culled path's limits) on the culled path.  The culled
+ * replicas' list building and sums run as one launch each for all of them. */
+lmg_status lmg_batch_create_with_path(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
+                                      const lmg_sim_state* states, int32_t device_id, int32_t de_path,
+                                      lmg_batch** out);
 void lmg_batch_destroy(lmg_batch* batch);
 const char* lmg_batch_last_error(const lmg_batch* batch);
 int32_t lmg_batch_size(const lmg_batch* batch);
 lmg_status lmg_batch_step(lmg_batch* batch, int32_t n_steps);
 /* as lmg_sim_set_graphs, for the batch step (graph replay is the default) */
 lmg_status lmg_batch_set_graphs(lmg_batch* batch, int32_t on);
-/* the common charge-chunk count of the batch's dE kernel, autotuned at
- * creation on the loaded replicas, and its time for all replicas (us) */
+/* the common charge-chunk count of the batch's all-pairs dE kernel (0 when
+ * every replica is culled), autotuned at creation on the loaded replicas, and
+ * the time of the whole dE phase for all replicas (us) */
 lmg_status lmg_batch_get_tuning(const lmg_batch* batch, int32_t* chunks, double* kernel_us);
+/* paths[r] = 1 (all pairs) or 2 (culled): the ΔE path replica r runs */
+lmg_status lmg_batch_get_de_paths(const lmg_batch* batch, int32_t* paths);
+/* every candidate the batch's autotuner timed, as a JSON object: lists of
+ * [shape, microseconds] for the culled workgroup totals ("cull_candidates"),
+ * the all-pairs chunk counts ("pairs_candidates", and "all_pairs_candidates"
+ * with every replica on that path), and the choice.  Valid until destroy. */
+const char* lmg_batch_tuning_report(const lmg_batch* batch);
 /* n steps timed with HIP events on the batch stream; *ms = event span */
 lmg_status lmg_batch_step_timed(lmg_batch* batch, int32_t n_steps, double* ms);
 /* n steps launched kernel by kernel with HIP events around each replica-wide dE

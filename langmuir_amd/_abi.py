@@ -116,6 +116,10 @@ def lib():
     L.lmg_sim_device_error.argtypes = [VP, PI32]
     L.lmg_sim_step_timed.argtypes = [VP, I32, PF64]
     L.lmg_batch_create.argtypes = [I32, VP, VP, VP, I32, C.POINTER(VP)]
+    L.lmg_batch_create_with_path.argtypes = [I32, VP, VP, VP, I32, I32, C.POINTER(VP)]
+    L.lmg_batch_get_de_paths.argtypes = [VP, C.POINTER(I32)]
+    L.lmg_batch_tuning_report.argtypes = [VP]
+    L.lmg_batch_tuning_report.restype = C.c_char_p
     L.lmg_batch_destroy.argtypes = [VP]
     L.lmg_batch_destroy.restype = None
     L.lmg_batch_last_error.argtypes = [VP]
@@ -159,7 +163,8 @@ EXPORTED = (
     "lmg_sim_set_de_chunks",
     "lmg_sim_get_de_path", "lmg_sim_delta_e",
     "lmg_sim_device_error", "lmg_sim_step_timed",
-    "lmg_batch_create", "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
+    "lmg_batch_create", "lmg_batch_create_with_path", "lmg_batch_get_de_paths", "lmg_batch_tuning_report",
+    "lmg_batch_destroy", "lmg_batch_last_error", "lmg_batch_size",
     "lmg_batch_step", "lmg_batch_step_timed", "lmg_batch_step_phases", "lmg_batch_delta_e", "lmg_batch_sync", "lmg_batch_store",
     "lmg_get_params",
     "lmg_surrogate_create", "lmg_surrogate_destroy", "lmg_surrogate_last_error", "lmg_surrogate_shape",

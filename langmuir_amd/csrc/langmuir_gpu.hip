@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -85,7 +86,9 @@ struct lmg_ctx {
     bool autotune = true;               // tune de_kernel at load (a batch tunes its replicas together)
     int forced_chunks = 0;              // lmg_sim_set_de_chunks (0: autotune)
     int tuned_chunks = 0;               // autotuned charge chunks of de_kernel (all-pairs path)
-    double tuned_us = -1.0;             // its measured dE-kernel time
+    double tuned_us = -1.0;             // its measured dE-kernel time (culled path: list building + sums)
+    int cull_resident = 0;              // cull_de_kernel workgroups resident on the device at once
+    int cull_quads_alloc = 0;           // work-item quads the culled path's item array holds
     hipGraphExec_t step_exec = nullptr; // one step, captured at the first lmg_sim_step after a load
     lmg_sim_config cfg{};
     SimDev sim{};
@@ -315,13 +318,15 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
     const SimDev& D = c->sim;
     const SimDev* P = c->d_sim;
     if (D.cull) {
-        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream, P);
+        const int2* none = nullptr;   // one replica: no workgroup map
+        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream,
+                           P, none);
         if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
         const dim3 grid((D.items_cap + 3) / 4 + 1);
         if (D.gauss)
-            hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P);
+            hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P, none);
         else
-            hipLaunchKernelGGL((cull_de_kernel<false>), grid, dim3(kSimBlock), 0, c->stream, P);
+            hipLaunchKernelGGL((cull_de_kernel<false>), grid, dim3(kSimBlock), 0, c->stream, P, none);
         return launch_check(c, "cull_de_kernel");
     }
     if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
@@ -358,21 +363,29 @@ lmg_status sim_launch_commit(lmg_ctx* c) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxChunks = 128;
 
+// mean time of `reps` launches after one untimed launch
 template <class F>
-double time_launch_us(hipStream_t st, F&& launch) {
+double time_launch_us(hipStream_t st, F&& launch, int reps = 3) {
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30;
     launch();
     (void)hipEventRecord(a, st);
-    for (int k = 0; k < 3; ++k) launch();
+    for (int k = 0; k < reps; ++k) launch();
     (void)hipEventRecord(b, st);
     float ms = 0.f;
     const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess &&
                     hipGetLastError() == hipSuccess;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return ok ? 1e3 * double(ms) / 3.0 : 1e30;
+    return ok ? 1e3 * double(ms) / double(reps) : 1e30;
 }
+
+// Autotuning of the culled path's work-item count: cull_de_kernel workgroups
+// (four chunks each) in multiples of what the device holds at once, k
+// resident waves of workgroups for k = 1 .. kCullTuneWaves; more workgroups
+// make smaller chunks, which balance better and cost more list scans.
+constexpr int kCullTuneWaves = 3;
+int cull_quads(int k, int resident, int ngrp) { return std::max(k * resident - 1, ngrp); }
 
 // candidates: powers of two up to kMaxChunks with at least one 32-charge block each
 std::vector<int> chunk_candidates(int n_charges) {
@@ -382,9 +395,36 @@ std::vector<int> chunk_candidates(int n_charges) {
     return v;
 }
 
+// The culled path's work-item count: candidates timed twice each (the second
+// round sees the clocks the first one raised), the fastest kept.  The list
+// building and the sums only rewrite per-step scratch, and the totals are
+// exact fixed-point sums of the same units for any item count.
+lmg_status sim_tune_cull(lmg_ctx* c) {
+    SimDev& D = c->sim;
+    c->tuned_us = -1.0;
+    if (c->forced_chunks > 0 || !c->autotune) return LMG_OK;
+    double best = 1e30;
+    int best_q = D.items_cap / 4;
+    for (int round = 0; round < 2; ++round)
+        for (int k = 1; k <= kCullTuneWaves; ++k) {
+            D.items_cap = 4 * cull_quads(k, c->cull_resident, D.ngrp);
+            HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+            const double us = time_launch_us(c->stream, [&] { (void)sim_launch_de(c); }, 5);
+            if (us < best) {
+                best = us;
+                best_q = D.items_cap / 4;
+            }
+        }
+    D.items_cap = 4 * best_q;
+    HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+    c->tuned_us = best;
+    return launch_check(c, "culled dE autotune");
+}
+
 lmg_status sim_tune_chunks(lmg_ctx* c) {
     SimDev& D = c->sim;
-    if (D.cull || !D.coulomb) return LMG_OK;
+    if (D.cull) return sim_tune_cull(c);
+    if (!D.coulomb) return LMG_OK;
     if (c->forced_chunks > 0) {
         D.S = std::min(c->forced_chunks, kMaxChunks);
         HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
@@ -749,12 +789,16 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                       &per_cu, reinterpret_cast<const void*>(&cull_de_kernel<false>), kSimBlock, 0));
-        // work items (workgroups of four chunks of one target group): as many
-        // as keep every workgroup of cull_de_kernel resident at once (one wave
-        // of workgroups, no tail), at least one per group; items_cap counts waves
-        // (lmg_sim_set_de_chunks fixes the workgroup count instead)
-        const int wgs = c->forced_chunks > 0 ? std::min(c->forced_chunks, kMaxCullWorkgroups) : ncu * per_cu - 1;
+        // work items (workgroups of four chunks of one target group): to start
+        // with, as many as keep every workgroup of cull_de_kernel resident at
+        // once (one wave of workgroups, no tail), at least one per group;
+        // items_cap counts waves.  The autotuner tries up to kCullTuneWaves
+        // waves of workgroups; lmg_sim_set_de_chunks fixes the count instead.
+        c->cull_resident = std::max(1, ncu * per_cu);
+        const int wgs = c->forced_chunks > 0 ? std::min(c->forced_chunks, kMaxCullWorkgroups)
+                                             : cull_quads(1, c->cull_resident, D.ngrp);
         D.items_cap = 4 * std::max(wgs, D.ngrp);
+        c->cull_quads_alloc = std::max(D.items_cap / 4, cull_quads(kCullTuneWaves, c->cull_resident, D.ngrp));
     }
 
     for (int b = 0; b < 2; ++b) {
@@ -785,7 +829,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
             (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
             (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
             (st = sim_alloc(c, D.torder, cap)) || (st = sim_alloc(c, D.tgrp, D.ngrp)) ||
-            (st = sim_alloc(c, D.items, D.items_cap)) || (st = sim_alloc(c, D.nitems, 1)) ||
+            (st = sim_alloc(c, D.items, 4 * size_t(c->cull_quads_alloc))) || (st = sim_alloc(c, D.nitems, 1)) ||
             (st = sim_alloc(c, D.isum, cap)))
             return st;
         HIPCHK(c, hipMemset(D.bmask, 0, size_t(nb) * 32 * 4));
@@ -944,7 +988,7 @@ lmg_status lmg_sim_get_tuning(const lmg_ctx* c, int32_t* shape, double* kernel_u
     if (!c || !shape || !kernel_us) return LMG_ERR_ARG;
     if (!c->sim_loaded) return LMG_ERR_STATE;
     *shape = c->sim.cull ? c->sim.items_cap : c->sim.S;
-    *kernel_us = c->sim.cull ? -1.0 : c->tuned_us;
+    *kernel_us = c->tuned_us;
     return LMG_OK;
 }
 
@@ -1118,10 +1162,22 @@ struct lmg_batch {
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<lmg_ctx*> reps;
-    SimDev* d_sims = nullptr;
-    int max_nblk = 0, max_S = 0;
-    int tuned_S = 0;          // autotuned common charge-chunk count
-    double tuned_us = -1.0;   // its dE-kernel time for all replicas
+    SimDev* d_sims = nullptr;   // every replica (the commit kernels)
+    int max_nblk = 0;
+    // all-pairs replicas: their descriptors, de_kernel's grid
+    SimDev* d_pairs = nullptr;
+    int n_pairs = 0, max_nblk_p = 0, max_S_p = 0;
+    // culled replicas: their descriptors and the workgroup maps of the two
+    // culled kernels (launch workgroup -> (culled replica, its workgroup))
+    SimDev* d_cull = nullptr;
+    int n_cull = 0;
+    int2* d_emit_map = nullptr;
+    int2* d_de_map = nullptr;
+    int emit_wgs = 0, de_wgs = 0, de_map_cap = 0;
+    std::vector<int> paths;   // per replica: 1 all pairs, 2 culled
+    int tuned_S = 0;          // autotuned common charge-chunk count of the all-pairs replicas
+    double tuned_us = -1.0;   // the dE phase of every replica at the chosen shapes
+    std::string report;       // every timed candidate (JSON)
     bool coulomb = false, gauss = false, any_recomb = false;
     bool use_graphs = true;
     hipGraphExec_t step_exec = nullptr;   // one step of every replica, captured at the first step
@@ -1145,14 +1201,27 @@ lmg_status bfail(lmg_batch* b, lmg_status code, const std::string& msg) {
             return bfail((b), LMG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// the dE kernel of every replica (pair sums, decision flags, random-stream refill)
+// the dE phase of every replica (pair sums, decision flags, random-stream
+// refill): the culled replicas' list building and sums, then de_kernel over the
+// all-pairs ones
 void batch_launch_de(lmg_batch* b) {
     const dim3 blk(kSimBlock);
-    const int R = int(b->reps.size());
-    const dim3 g_de(b->max_nblk + 1, b->coulomb ? b->max_S : 1, R);
-    if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_sims);
-    else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_sims);
-    else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_sims);
+    if (b->n_cull) {
+        hipLaunchKernelGGL(cull_emit_kernel, dim3(b->emit_wgs), blk, 0, b->stream, b->d_cull,
+                           (const int2*)b->d_emit_map);
+        if (b->gauss)
+            hipLaunchKernelGGL((cull_de_kernel<true>), dim3(b->de_wgs), blk, 0, b->stream, b->d_cull,
+                               (const int2*)b->d_de_map);
+        else
+            hipLaunchKernelGGL((cull_de_kernel<false>), dim3(b->de_wgs), blk, 0, b->stream, b->d_cull,
+                               (const int2*)b->d_de_map);
+    }
+    if (b->n_pairs) {
+        const dim3 g_de(b->max_nblk_p + 1, b->coulomb ? b->max_S_p : 1, b->n_pairs);
+        if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_pairs);
+        else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_pairs);
+        else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_pairs);
+    }
 }
 
 // decisions, recombination, ticks, injection and the next proposals of every replica
@@ -1192,6 +1261,230 @@ lmg_status batch_steps(lmg_batch* b, int n) {
     return LMG_OK;
 }
 
+
+// the single context's geometric test for the culled path: the cutoff box
+// around a group of targets (about 16 sites wide) covers well under the lattice
+bool cull_candidate(const lmg_ctx* c) {
+    auto frac = [&](int n) { return std::min(1.0, double(16 + 2 * c->p.cutoff) / n); };
+    return frac(c->d.X) * frac(c->d.Y) * frac(c->d.Z) < 0.6;
+}
+
+// Puts a plan into the batch's descriptors and workgroup maps: culled[r] for the
+// replicas on the culled path (with quads[r] work-item quads), the others on the
+// all-pairs path with S charge chunks (capped per replica).  The replicas' own
+// descriptors follow, so that their single-context calls (lmg_batch_delta_e)
+// take the same path.
+lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, const std::vector<int>& quads) {
+    const int n = int(b->reps.size());
+    std::vector<SimDev> all(n), pairs, cull;
+    std::vector<int> cull_rep;
+    b->max_nblk_p = 0;
+    b->max_S_p = 1;
+    for (int r = 0; r < n; ++r) {
+        lmg_ctx* c = b->reps[r];
+        SimDev& D = c->sim;
+        D.cull = culled[r] ? 1 : 0;
+        if (D.cull) {
+            D.items_cap = 4 * quads[r];
+            cull.push_back(D);
+            cull_rep.push_back(r);
+        } else {
+            D.S = std::max(1, std::min(D.S_pairs, S));
+            pairs.push_back(D);
+            b->max_nblk_p = std::max(b->max_nblk_p, D.nblk);
+            b->max_S_p = std::max(b->max_S_p, D.S);
+        }
+        all[r] = D;
+        b->paths[r] = D.cull ? 2 : 1;
+        BHIPCHK(b, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
+    }
+    b->n_pairs = int(pairs.size());
+    b->n_cull = int(cull.size());
+    BHIPCHK(b, hipMemcpy(b->d_sims, all.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice));
+    if (b->n_pairs) BHIPCHK(b, hipMemcpy(b->d_pairs, pairs.data(), sizeof(SimDev) * b->n_pairs, hipMemcpyHostToDevice));
+    if (!b->n_cull) return LMG_OK;
+    BHIPCHK(b, hipMemcpy(b->d_cull, cull.data(), sizeof(SimDev) * b->n_cull, hipMemcpyHostToDevice));
+    // cull_emit_kernel: replica by replica, ceil(nbrick / 4) + nblk + 1 workgroups each
+    std::vector<int2> emit, de;
+    for (int j = 0; j < b->n_cull; ++j) {
+        const SimDev& D = cull[j];
+        for (int x = 0, e = (D.nbrick + 3) / 4 + D.nblk + 1; x < e; ++x) emit.push_back(make_int2(j, x));
+    }
+    // cull_de_kernel: every replica's refill workgroup first (they run the
+    // longest serial loop), then the work items, the replicas with the most
+    // work per item first, so that the last workgroups to start are short ones
+    std::vector<std::pair<double, int>> order;
+    for (int j = 0; j < b->n_cull; ++j) {
+        const SimDev& D = cull[j];
+        de.push_back(make_int2(j, D.items_cap / 4));
+        const double cap = D.cap_e + D.cap_h + D.n_d_eff;
+        order.push_back({-cap * cap / double(D.items_cap / 4), j});
+    }
+    std::sort(order.begin(), order.end());
+    for (const auto& o : order)
+        for (int x = 0; x < cull[o.second].items_cap / 4; ++x) de.push_back(make_int2(o.second, x));
+    const int need = int(std::max(emit.size(), de.size()));
+    if (need > b->de_map_cap) {
+        if (b->d_emit_map) (void)hipFree(b->d_emit_map);
+        if (b->d_de_map) (void)hipFree(b->d_de_map);
+        b->d_emit_map = b->d_de_map = nullptr;
+        BHIPCHK(b, hipMalloc(&b->d_emit_map, sizeof(int2) * need));
+        BHIPCHK(b, hipMalloc(&b->d_de_map, sizeof(int2) * need));
+        b->de_map_cap = need;
+    }
+    BHIPCHK(b, hipMemcpy(b->d_emit_map, emit.data(), sizeof(int2) * emit.size(), hipMemcpyHostToDevice));
+    BHIPCHK(b, hipMemcpy(b->d_de_map, de.data(), sizeof(int2) * de.size(), hipMemcpyHostToDevice));
+    b->emit_wgs = int(emit.size());
+    b->de_wgs = int(de.size());
+    return LMG_OK;
+}
+
+// The batch's autotuner.  Candidates, each timed twice (the second round sees
+// the clocks the first one raised) and scored by its faster time:
+//   culled replicas: the total number of cull_de_kernel workgroups (one per
+//     target group at least, then 1 .. 4 device-resident waves of workgroups,
+//     or 1.5 and 2 per target group), shared out in proportion to each
+//     replica's estimated pair count;
+//   all-pairs replicas: the common charge-chunk count S (powers of two);
+//   automatic path: the same S search with every replica on the all-pairs
+//     path, against the best culled split.
+// A dE launch only rewrites per-step scratch, and neither the path nor the
+// shapes change a replica's results beyond the FP32 rounding of its unit sums
+// (bit-identical to its single context on the same path).
+lmg_status batch_plan(lmg_batch* b, int de_path) {
+    const int n = int(b->reps.size());
+    b->paths.assign(n, 1);
+    std::vector<char> capable(n, 0);
+    int max_charges = 1;
+    for (int r = 0; r < n; ++r) {
+        const SimDev& D = b->reps[r]->sim;
+        capable[r] = D.cull ? 1 : 0;
+        b->max_nblk = std::max(b->max_nblk, D.nblk);
+        b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
+        max_charges = std::max(max_charges, D.cap_e + D.cap_h + D.n_d_eff);
+    }
+    BHIPCHK(b, hipMalloc(&b->d_sims, sizeof(SimDev) * n));
+    BHIPCHK(b, hipMalloc(&b->d_pairs, sizeof(SimDev) * n));
+    BHIPCHK(b, hipMalloc(&b->d_cull, sizeof(SimDev) * n));
+    const std::vector<char> none(n, 0);
+    const bool any_cull = std::find(capable.begin(), capable.end(), 1) != capable.end();
+    std::ostringstream js;
+    js << "{";
+    auto timed = [&](double& best) {
+        const double us = !b->coulomb ? 0.0 : time_launch_us(b->stream, [&] { batch_launch_de(b); }, 5);
+        best = std::min(best, us);
+        return us;
+    };
+
+    // culled replicas: total work-item quads
+    std::vector<int> quads(n, 0);
+    double cull_us = 1e30;
+    if (any_cull) {
+        long long groups = 0, alloc = 0, ncap = 0;
+        double wsum = 0.0;
+        std::vector<double> w(n, 0.0);
+        int resident = 1;
+        for (int r = 0; r < n; ++r) {
+            if (!capable[r]) continue;
+            const lmg_ctx* c = b->reps[r];
+            const SimDev& D = c->sim;
+            groups += D.ngrp;
+            alloc += c->cull_quads_alloc;
+            ++ncap;
+            resident = std::max(resident, c->cull_resident);
+            auto frac = [&](int e) { return std::min(1.0, double(16 + 2 * c->p.cutoff) / e); };
+            const double cap = D.cap_e + D.cap_h + D.n_d_eff;
+            w[r] = cap * cap * frac(c->d.X) * frac(c->d.Y) * frac(c->d.Z);
+            wsum += w[r];
+        }
+        std::vector<long long> cand;
+        for (int k = 1; k <= 4; ++k) cand.push_back(std::max(groups, (long long)k * resident - ncap));
+        cand.push_back((3 * groups + 1) / 2);
+        cand.push_back(2 * groups);
+        for (long long& t : cand) t = std::min(t, alloc);
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        auto share = [&](long long total, std::vector<int>& q) {
+            const double extra = double(total - groups);
+            for (int r = 0; r < n; ++r) {
+                if (!capable[r]) continue;
+                const lmg_ctx* c = b->reps[r];
+                const int want = c->sim.ngrp + int(extra * w[r] / std::max(wsum, 1e-30));
+                q[r] = std::max(c->sim.ngrp, std::min(want, c->cull_quads_alloc));
+            }
+        };
+        std::vector<double> best(cand.size(), 1e30);
+        for (int round = 0; round < 2; ++round)
+            for (size_t k = 0; k < cand.size(); ++k) {
+                share(cand[k], quads);
+                lmg_status st = batch_apply(b, capable, 1, quads);
+                if (st != LMG_OK) return st;
+                b->n_pairs = 0;   // the culled replicas alone
+                timed(best[k]);
+            }
+        size_t kb = 0;
+        js << "\"cull_candidates\": [";
+        for (size_t k = 0; k < cand.size(); ++k) {
+            js << (k ? ", " : "") << "[" << cand[k] << ", " << best[k] << "]";
+            if (best[k] < best[kb]) kb = k;
+        }
+        js << "], ";
+        share(cand[kb], quads);
+        cull_us = best[kb];
+    }
+
+    // all-pairs replicas: S, with the cullable ones culled (split) or not (all pairs)
+    auto tune_s = [&](const std::vector<char>& culled, const char* name, int& best_S) {
+        const std::vector<int> cands = chunk_candidates(std::min(max_charges, 32 * 64));   // up to 64 chunks
+        std::vector<double> best(cands.size(), 1e30);
+        for (int round = 0; round < 2; ++round)
+            for (size_t k = 0; k < cands.size(); ++k) {
+                if (batch_apply(b, culled, cands[k], quads) != LMG_OK) return -1.0;
+                b->n_cull = 0;   // the all-pairs replicas alone
+                timed(best[k]);
+                if (!b->coulomb) break;
+            }
+        size_t kb = 0;
+        js << "\"" << name << "\": [";
+        for (size_t k = 0; k < cands.size(); ++k) {
+            js << (k ? ", " : "") << "[" << cands[k] << ", " << best[k] << "]";
+            if (best[k] < best[kb]) kb = k;
+        }
+        js << "], ";
+        best_S = cands[kb];
+        return best[kb];
+    };
+    std::vector<char> plan = de_path == 1 ? none : capable;
+    int S_split = 1, S_all = 1;
+    double split_us = any_cull && de_path != 1 ? cull_us : 0.0;
+    const bool rest = std::find(plan.begin(), plan.end(), 0) != plan.end();
+    if (rest) {
+        const double t = tune_s(plan, "pairs_candidates", S_split);
+        if (t < 0) return b->err.empty() ? bfail(b, LMG_ERR_HIP, "batch autotune failed") : LMG_ERR_HIP;
+        split_us += t;
+    }
+    int S = S_split;
+    if (de_path == 0 && any_cull) {   // the same replicas all on the all-pairs path
+        const double all_us = tune_s(none, "all_pairs_candidates", S_all);
+        if (all_us < 0) return b->err.empty() ? bfail(b, LMG_ERR_HIP, "batch autotune failed") : LMG_ERR_HIP;
+        if (all_us < split_us) {
+            plan = none;
+            S = S_all;
+            split_us = all_us;
+        }
+    }
+    const lmg_status st = batch_apply(b, plan, S, quads);
+    if (st != LMG_OK) return st;
+    b->tuned_S = b->n_pairs ? S : 0;
+    b->tuned_us = b->coulomb ? split_us : -1.0;
+    long long qt = 0;
+    for (int r = 0; r < n; ++r) qt += plan[r] ? quads[r] : 0;
+    js << "\"culled_replicas\": " << b->n_cull << ", \"cull_workgroups\": " << qt
+       << ", \"charge_chunks\": " << b->tuned_S << ", \"de_phase_us\": " << b->tuned_us << "}";
+    b->report = js.str();
+    return LMG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1204,7 +1497,8 @@ void lmg_batch_destroy(lmg_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->step_exec) (void)hipGraphExecDestroy(b->step_exec);
     for (lmg_ctx* c : b->reps) lmg_destroy(c);
-    if (b->d_sims) (void)hipFree(b->d_sims);
+    for (void* p : {(void*)b->d_sims, (void*)b->d_pairs, (void*)b->d_cull, (void*)b->d_emit_map, (void*)b->d_de_map})
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1212,100 +1506,51 @@ void lmg_batch_destroy(lmg_batch* b) {
 
 lmg_status lmg_batch_create(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
                             const lmg_sim_state* states, int32_t device_id, lmg_batch** out) {
+    return lmg_batch_create_with_path(n, params, configs, states, device_id, 0, out);
+}
+
+lmg_status lmg_batch_create_with_path(int32_t n, const lmg_params* params, const lmg_sim_config* configs,
+                                      const lmg_sim_state* states, int32_t device_id, int32_t de_path,
+                                      lmg_batch** out) {
     if (!out || n < 1 || !params || !configs || !states) return bfail(nullptr, LMG_ERR_ARG, "invalid batch arguments");
+    if (de_path < 0 || de_path > 2)
+        return bfail(nullptr, LMG_ERR_ARG, "dE path must be 0 (automatic), 1 (all pairs) or 2 (culled)");
     *out = nullptr;
     lmg_batch* b = new lmg_batch;
     b->dev = device_id;
-    long long total_waves = 0;
+    auto give_up = [&](lmg_status st, const std::string& msg) {
+        bfail(nullptr, st, msg);
+        lmg_batch_destroy(b);
+        return st;
+    };
     for (int r = 0; r < n; ++r) {
         lmg_params p = params[r];
         p.device_id = device_id;
         lmg_ctx* c = nullptr;
         lmg_status st = lmg_create(&p, &c);
-        if (st != LMG_OK) {
-            bfail(nullptr, st, "replica " + std::to_string(r) + ": " + lmg_last_error(nullptr));
-            lmg_batch_destroy(b);
-            return st;
-        }
+        if (st != LMG_OK) return give_up(st, "replica " + std::to_string(r) + ": " + lmg_last_error(nullptr));
         b->reps.push_back(c);
         c->autotune = false;
+        // the culled path for the replicas that can use it (de_path 2), or that
+        // the single context's geometric test picks (automatic)
+        c->de_mode = de_path == 2 || (de_path == 0 && cull_candidate(c)) ? 2 : 1;
         st = lmg_sim_load(c, &configs[r], &states[r]);
-        if (st != LMG_OK) {
-            bfail(nullptr, st, "replica " + std::to_string(r) + ": " + lmg_last_error(c));
-            lmg_batch_destroy(b);
-            return st;
-        }
+        if (st != LMG_OK) return give_up(st, "replica " + std::to_string(r) + ": " + lmg_last_error(c));
         const SimDev& D = c->sim;
         if (r == 0) {
             b->coulomb = D.coulomb;
             b->gauss = D.gauss;
         } else if (b->coulomb != bool(D.coulomb) || (D.coulomb && b->gauss != bool(D.gauss))) {
-            bfail(nullptr, LMG_ERR_ARG, "replicas must agree on coulomb.carriers and Gaussian screening");
-            lmg_batch_destroy(b);
-            return LMG_ERR_ARG;
+            return give_up(LMG_ERR_ARG, "replicas must agree on coulomb.carriers and Gaussian screening");
         }
-        total_waves += (D.cap_e + D.cap_h + 63) / 64;
     }
-    if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        bfail(nullptr, LMG_ERR_HIP, "batch stream creation failed");
-        lmg_batch_destroy(b);
-        return LMG_ERR_HIP;
+    if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return give_up(LMG_ERR_HIP, "batch stream creation failed");
+    const lmg_status st = batch_plan(b, de_path);
+    if (st != LMG_OK) {
+        const std::string msg = b->err;
+        return give_up(st, msg);
     }
-    (void)total_waves;
-    // every replica runs the all-pairs dE kernel of one launch for all replicas; the common
-    // charge-chunk count S (capped per replica by its own) is autotuned on the loaded states
-    std::vector<SimDev> host(n);
-    int max_charges = 1;
-    for (int r = 0; r < n; ++r) {
-        SimDev& D = b->reps[r]->sim;
-        D.cull = 0;
-        b->max_nblk = std::max(b->max_nblk, D.nblk);
-        b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
-        max_charges = std::max(max_charges, D.cap_e + D.cap_h + D.n_d_eff);
-    }
-    if (hipMalloc(&b->d_sims, sizeof(SimDev) * n) != hipSuccess) {
-        bfail(nullptr, LMG_ERR_HIP, "descriptor array allocation failed");
-        lmg_batch_destroy(b);
-        return LMG_ERR_HIP;
-    }
-    auto apply = [&](int S) -> bool {
-        b->max_S = 1;
-        for (int r = 0; r < n; ++r) {
-            SimDev& D = b->reps[r]->sim;
-            D.S = std::max(1, std::min(D.S_pairs, S));
-            host[r] = D;
-            b->max_S = std::max(b->max_S, D.S);
-            // the single-context descriptor follows the same chunking
-            if (hipMemcpy(b->reps[r]->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice) != hipSuccess) return false;
-        }
-        return hipMemcpy(b->d_sims, host.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice) == hipSuccess;
-    };
-    double best = 1e30;
-    int best_S = 1;
-    for (int S : chunk_candidates(std::min(max_charges, 32 * 64))) {   // up to 64 chunks
-        if (!apply(S)) {
-            bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
-            lmg_batch_destroy(b);
-            return LMG_ERR_HIP;
-        }
-        const double us = !b->coulomb ? 0.0 : time_launch_us(b->stream, [&] {
-            const dim3 g_de(b->max_nblk + 1, b->max_S, n);
-            if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, dim3(kSimBlock), 0, b->stream, b->d_sims);
-            else hipLaunchKernelGGL((de_kernel<false, true>), g_de, dim3(kSimBlock), 0, b->stream, b->d_sims);
-        });
-        if (us < best) {
-            best = us;
-            best_S = S;
-        }
-        if (!b->coulomb) break;
-    }
-    if (!apply(best_S)) {
-        bfail(nullptr, LMG_ERR_HIP, "descriptor upload failed");
-        lmg_batch_destroy(b);
-        return LMG_ERR_HIP;
-    }
-    b->tuned_S = best_S;
-    b->tuned_us = b->coulomb ? best : -1.0;
     *out = b;
     return LMG_OK;
 }
@@ -1316,6 +1561,14 @@ lmg_status lmg_batch_get_tuning(const lmg_batch* b, int32_t* chunks, double* ker
     *kernel_us = b->tuned_us;
     return LMG_OK;
 }
+
+lmg_status lmg_batch_get_de_paths(const lmg_batch* b, int32_t* paths) {
+    if (!b || !paths) return LMG_ERR_ARG;
+    for (size_t r = 0; r < b->paths.size(); ++r) paths[r] = b->paths[r];
+    return LMG_OK;
+}
+
+const char* lmg_batch_tuning_report(const lmg_batch* b) { return b ? b->report.c_str() : ""; }
 
 lmg_status lmg_batch_step(lmg_batch* b, int32_t n) {
     if (!b) return LMG_ERR_ARG;

@@ -517,52 +517,81 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, con
 }
 
 // Work items of cull_de_kernel, one workgroup of kSimBlock: group g gets
-// S_g = max(1, round(K T_g / sum T)) chunks, T_g its slots in the previous
-// culled pass (the groups' regions move by about a site per step), K the
-// number of item slots left after one item per group; items are (g, chunk,
-// S_g) in group order.  Any S_g gives the same sums: it only balances the work.
+// Q_g = 1 + floor(K C_{g+1} / W) - floor(K C_g / W) workgroups of four chunks,
+// C_g the sum of T over the groups before g, T_g its slots in the previous
+// culled pass (the groups' regions move by about a site per step), W the sum of
+// all T_g and K the workgroups left after one per group: every item slot is
+// used, and each group's share is K T_g / W rounded up or down.  Items are
+// (g, quad, Q_g) in group order.  Any Q_g gives the same sums: it only
+// balances the work.
 __device__ void cull_items(const SimDev& D, int* wsum) {
     const int ng = (D.ctl->counts[0] + kCullGroup - 1) / kCullGroup;
     const int per = (ng + kSimBlock - 1) / kSimBlock;
     const int g0 = min(ng, int(threadIdx.x) * per), g1 = min(ng, g0 + per);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     long long wl = 0;
     for (int g = g0; g < g1; ++g) wl += D.tgrp[g];
-    // sum T over the workgroup (wave sums, then the four waves)
-    for (int o = 32; o > 0; o >>= 1) wl += __shfl_xor(wl, o, 64);
+    // exclusive scan of the threads' sums over the workgroup (64-bit)
+    long long x = wl;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
     __shared__ long long wtot[4];
-    if ((threadIdx.x & 63) == 0) wtot[threadIdx.x >> 6] = wl;
+    if (lane == 63) wtot[wid] = x;
     __syncthreads();
+    long long before = 0;
+    for (int w = 0; w < wid; ++w) before += wtot[w];
     const long long W = max(1ll, wtot[0] + wtot[1] + wtot[2] + wtot[3]);
-    // workgroups of four chunks: one per group, the rest shared in proportion to T
     const long long K = max(0, D.items_cap / 4 - ng);
-    auto quads = [&](int g) { return 1 + int((K * D.tgrp[g]) / W); };
+    const long long c0 = before + x - wl;
+    auto cut = [&](long long c) { return (K * c) / W; };
     int run = 0;
-    for (int g = g0; g < g1; ++g) run += quads(g);
+    long long c = c0;
+    for (int g = g0; g < g1; ++g) {
+        const long long t = D.tgrp[g];
+        run += 1 + int(cut(c + t) - cut(c));
+        c += t;
+    }
     int total;
     int k = wg_scan<kSimBlock>(run, wsum, total);
+    c = c0;
     for (int g = g0; g < g1; ++g) {
-        const int qg = quads(g);
-        for (int c = 0; c < qg; ++c) D.items[k + c] = make_int4(g, c, qg, 0);
+        const long long t = D.tgrp[g];
+        const int qg = 1 + int(cut(c + t) - cut(c));
+        c += t;
+        for (int q = 0; q < qg; ++q) D.items[k + q] = make_int4(g, q, qg, 0);
         k += qg;
     }
     if (threadIdx.x == 0) *D.nitems = total;
 }
 
-// grid = ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
-__global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+// The culled kernels of many replicas run as one launch: workgroup x of the
+// launch is workgroup map[x].y of replica map[x].x (descriptor Dp[map[x].x]).
+// A single context passes no map (replica 0, workgroup blockIdx.x).
+__device__ __forceinline__ int2 cull_workgroup(const int2* __restrict__ map) {
+    return map ? map[blockIdx.x] : make_int2(0, int(blockIdx.x));
+}
+
+// per replica ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
+__global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp,
+                                                              const int2* __restrict__ map) {
+    const int2 wg = cull_workgroup(map);
+    const SimDev& D = Dp[wg.x];
+    const int wx = wg.y;
     const int* sst = D.sstart;
     const int* tst = D.tstart;
     const int nbw = (D.nbrick + 3) / 4;
-    if (blockIdx.x == gridDim.x - 1) {   // cull_de_kernel's work items
+    if (wx == nbw + D.nblk) {   // cull_de_kernel's work items
         __shared__ int wsum[4];
         cull_items(D, wsum);
         return;
     }
-    if (int(blockIdx.x) >= nbw) {   // targets: proposals, decision-draw flags, their brick's slots
+    if (wx >= nbw) {   // targets: proposals, decision-draw flags, their brick's slots
         const SimCtl& C = *D.ctl;
         const int ne = C.n_e, nt = C.counts[0];
-        const int i = (blockIdx.x - nbw) * kSimBlock + threadIdx.x;
+        const int i = (wx - nbw) * kSimBlock + threadIdx.x;
         int draw = 0;
         if (i < nt) {
             const bool isE = i < ne;
@@ -580,10 +609,10 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
             D.torder[tst[tp.x] + tp.y] = i;
         }
         const int cnt = __syncthreads_count(draw);
-        if (threadIdx.x == 0) D.blk_draw[blockIdx.x - nbw] = cnt;
+        if (threadIdx.x == 0) D.blk_draw[wx - nbw] = cnt;
         return;
     }
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int b = wx * 4 + (threadIdx.x >> 6);
     if (b >= D.nbrick) return;
     const int lane = threadIdx.x & 63;
     const uint32_t v = lane < 32 ? D.bmask[b * 32 + lane] : 0u;
@@ -657,7 +686,7 @@ constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
 
-// grid = ceil(items_cap / 4) + 1 workgroups: workgroup x takes work item x
+// Per replica ceil(items_cap / 4) + 1 workgroups: workgroup x takes work item x
 // (group g, quad q of Q) and its wave w chunk s = 4 q + w of S = 4 Q; the last
 // workgroup refills the random stream.
 //
@@ -679,18 +708,21 @@ constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's lis
 // targets; integer addition is exact, so each target's total is the same
 // whatever its group, chunk count or order.
 template <bool kGauss>
-__global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp) {
-    const SimDev& D = *Dp;
+__global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp,
+                                                            const int2* __restrict__ map) {
+    const int2 wg = cull_workgroup(map);
+    const SimDev& D = Dp[wg.x];
+    const int wx = wg.y;
     __shared__ float4 lists[4][kList];
     __shared__ int uends[4][kList];   // (start << 16) | end of each unit in the list
-    if (blockIdx.x == gridDim.x - 1) {
+    if (wx == (D.items_cap + 3) / 4) {
         mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     const int nt = D.ctl->counts[0];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (int(blockIdx.x) >= *D.nitems) return;   // whole workgroup
-    const int4 it = D.items[blockIdx.x];
+    if (wx >= *D.nitems) return;   // whole workgroup
+    const int4 it = D.items[wx];
     const int g = it.x, s = 4 * it.y + wv, S = 4 * it.z;
     const int r = g * kCullGroup + lane;
     const bool valid = r < nt;

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import json
 import math
 from typing import Dict, Iterable, Optional, Sequence
 
@@ -489,7 +490,13 @@ class Batch:
     the keyword arguments of :func:`pack_state` (electrons, holes, potential, ...).
     """
 
-    def __init__(self, replicas, device_id: int = 0):
+    def __init__(self, replicas, device_id: int = 0, de_path: str = "auto"):
+        """``de_path``: "auto" (the autotuner picks the culled path for the replicas that
+        suit it, or none), "pairs" (every replica all-pairs) or "culled" (every replica
+        that can be culled)."""
+        modes = {"auto": 0, "pairs": 1, "culled": 2}
+        if de_path not in modes:
+            raise ValueError(f"de_path must be one of {sorted(modes)}")
         L = _abi.lib()
         n = len(replicas)
         self._params = (LmgParams * n)()
@@ -509,8 +516,9 @@ class Batch:
             self.caps.append((max(self._configs[r].max_electrons, st.n_electrons, 1),
                               max(self._configs[r].max_holes, st.n_holes, 1)))
         h = C.c_void_p()
-        rc = L.lmg_batch_create(n, C.cast(self._params, C.c_void_p), C.cast(self._configs, C.c_void_p),
-                                C.cast(self._states, C.c_void_p), device_id, C.byref(h))
+        rc = L.lmg_batch_create_with_path(n, C.cast(self._params, C.c_void_p), C.cast(self._configs, C.c_void_p),
+                                          C.cast(self._states, C.c_void_p), device_id, modes[de_path],
+                                          C.byref(h))
         if rc != 0:
             raise LmgError(rc, L.lmg_batch_last_error(None).decode())
         self._h = h
@@ -565,10 +573,23 @@ class Batch:
 
     @property
     def tuning(self) -> dict:
-        """The autotuned common charge-chunk count of the dE kernel and its time for all replicas."""
+        """What the autotuner chose and every candidate it timed: the all-pairs replicas'
+        common charge-chunk count (0 when all are culled), the culled replicas' workgroup
+        total, the dE phase's time for all replicas, and [shape, us] lists."""
         s, us = C.c_int32(0), C.c_double(0.0)
-        self._chk(_abi.lib().lmg_batch_get_tuning(self._h, C.byref(s), C.byref(us)))
-        return {"charge_chunks": int(s.value), "de_kernel_us": float(us.value)}
+        L = _abi.lib()
+        self._chk(L.lmg_batch_get_tuning(self._h, C.byref(s), C.byref(us)))
+        out = json.loads(L.lmg_batch_tuning_report(self._h).decode())
+        out.update(charge_chunks=int(s.value), de_kernel_us=float(us.value))
+        return out
+
+    @property
+    def de_paths(self) -> list:
+        """Per replica the dE path it runs: "pairs" or "culled"."""
+        n = len(self)
+        paths = (C.c_int32 * n)()
+        self._chk(_abi.lib().lmg_batch_get_de_paths(self._h, paths))
+        return ["culled" if v == 2 else "pairs" for v in paths]
 
     def state(self, replica: int) -> dict:
         st, bufs = _store_buffers(*self.caps[replica])
@@ -636,12 +657,13 @@ class Simulation:
 
     @property
     def tuning(self) -> dict:
-        """The dE kernel's shape chosen at load: charge chunks and measured time (all pairs),
-        or work items (culled)."""
+        """The dE kernel's shape chosen at load and its measured time (-1 when forced):
+        charge chunks (all pairs) or work items (culled; the time covers the list
+        building)."""
         shape, us = C.c_int32(0), C.c_double(0.0)
         self._chk(_abi.lib().lmg_sim_get_tuning(self.ctx.handle, C.byref(shape), C.byref(us)))
         if self.de_path == "culled":
-            return {"path": "culled", "work_items": int(shape.value)}
+            return {"path": "culled", "work_items": int(shape.value), "de_phase_us": float(us.value)}
         return {"path": "pairs", "charge_chunks": int(shape.value), "de_kernel_us": float(us.value)}
 
     def perform_iterations(self, n: int):

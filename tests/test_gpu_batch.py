@@ -22,7 +22,7 @@ BASE = dict(coulomb_carriers=1, electron_percentage=0.0, seed_charges=1.0, volta
             e_source_l_rate=0.0, h_source_l_rate=0.9, h_drain_r_rate=0.9)
 
 
-def build(oracle, gpu):
+def build(oracle, gpu, de_path="auto"):
     worlds, reps = [], []
     for kw in REPLICAS:
         k = dict(BASE)
@@ -34,13 +34,23 @@ def build(oracle, gpu):
         st = oracle_state(w)
         worlds.append((w, sp, st))
         reps.append(dict(params=sp, **st))
-    return worlds, gpu.Batch(reps)
+    return worlds, gpu.Batch(reps, de_path=de_path)
 
 
-def test_batch_matches_single_contexts_and_oracle(gpu, oracle):
-    worlds, batch = build(oracle, gpu)
+@pytest.mark.parametrize("de_path", ["auto", "culled"])
+def test_batch_matches_single_contexts_and_oracle(gpu, oracle, de_path):
+    """40 steps of every replica against its single context on the same dE path and
+    against the oracle; "culled" puts every replica through the batched culled kernels
+    (one launch for all of them, workgroups mapped to replicas)."""
+    worlds, batch = build(oracle, gpu, de_path)
     assert len(batch) == len(REPLICAS)
-    singles = [gpu.Simulation(sp, **st) for (_, sp, st) in worlds]
+    paths = batch.de_paths
+    print(f"batch dE paths {paths}, tuning {batch.tuning}", flush=True)
+    if de_path == "culled":
+        assert paths == ["culled"] * len(REPLICAS)
+    else:   # none of these small lattices passes the geometric test
+        assert paths == ["pairs"] * len(REPLICAS)
+    singles = [gpu.Simulation(sp, de_path=paths[r], **st) for r, (_, sp, st) in enumerate(worlds)]
     for t in range(40):
         batch.step(1)
         for s in singles:
@@ -61,6 +71,15 @@ def test_batch_autotuned_chunks(gpu, oracle):
     t = batch.tuning
     assert t["charge_chunks"] >= 1 and t["charge_chunks"] & (t["charge_chunks"] - 1) == 0
     assert t["de_kernel_us"] > 0
+    cands = [S for S, _ in t["pairs_candidates"]]
+    assert cands == [2 ** k for k in range(len(cands))] and len(cands) >= 2
+    assert min(us for _, us in t["pairs_candidates"]) == pytest.approx(t["de_phase_us"])
+    _, culled = build(oracle, gpu, "culled")
+    t = culled.tuning
+    assert t["charge_chunks"] == 0 and t["culled_replicas"] == len(REPLICAS)
+    assert all(q >= 1 for q, _ in t["cull_candidates"]) and t["cull_workgroups"] >= 1
+    assert t["de_phase_us"] == pytest.approx(min(us for _, us in t["cull_candidates"]))
+    print(f"culled batch tuning {t}", flush=True)
 
 
 def test_batch_determinism_and_invariants(gpu, oracle):

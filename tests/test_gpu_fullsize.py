@@ -9,7 +9,8 @@
   32^3 .. 128x128x16, 100 .. 5000 holes, 0 .. 10 % traps with depths
   clip(N(0.3, 0.1), 0.1, 0.5) eV, cutoffs 20 .. 50, own seeds), including the largest
   lattice and 5000 holes: 10 steps, every replica after every step equal to its own
-  single-context run and to the oracle.
+  single-context run on the dE path the batch gave it and to the oracle -- once with
+  the paths the autotuner picks, once with every replica culled.
 
 A step may differ from the oracle only through a Metropolis decision inside the band
 the dE tolerance opens (as in test_gpu_sim); the solar run re-synchronises then, a
@@ -89,10 +90,11 @@ def bench_like_replicas(n=16, seed=4):
     return out
 
 
-def test_batch_of_bench_replicas_10_steps(gpu, oracle):
+@pytest.mark.parametrize("de_path", ["auto", "culled"])
+def test_batch_of_bench_replicas_10_steps(gpu, oracle, de_path):
     from test_gpu_sim import run_parity  # noqa: F401  (same near-threshold rule below)
     reps = bench_like_replicas()
-    worlds, singles, batch_in = [], [], []
+    worlds, states, batch_in = [], [], []
     for r, rp in enumerate(reps):
         X, Y, Z = rp["dims"]
         V = X * Y * Z
@@ -106,11 +108,17 @@ def test_batch_of_bench_replicas_10_steps(gpu, oracle):
         assert np.array_equal(pot, w.potential())
         st = oracle_state(w)
         worlds.append((w, sp))
-        singles.append(gpu.Simulation(sp, de_path="pairs", **st))
+        states.append(st)
         batch_in.append(dict(params=sp, **st))
     print(f"batch of {len(reps)}: lattices {[r['dims'] for r in reps[:4]]} ..., "
           f"holes {sum(len(r['holes']) for r in reps)}", flush=True)
-    batch = gpu.Batch(batch_in)
+    batch = gpu.Batch(batch_in, de_path=de_path)
+    paths = batch.de_paths
+    print(f"batch dE paths {paths}", flush=True)
+    if de_path == "culled":
+        assert paths[0] == "culled" and paths.count("culled") == len(paths)
+    # each replica's single context on the path the batch gave it (same unit sums, same bits)
+    singles = [gpu.Simulation(sp, de_path=paths[r], **st) for r, ((_, sp), st) in enumerate(zip(worlds, states))]
     diverged = set()
     for t in range(10):
         batch.step(1)
