@@ -824,8 +824,8 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         int *bperm = nullptr, *dstart = nullptr;
         float4* dsrc = nullptr;
         if ((st = sim_alloc(c, D.bmask, size_t(nb) * 32)) || (st = sim_alloc(c, D.scnt, nb)) ||
-            (st = sim_alloc(c, D.tcnt, nb)) || (st = sim_alloc(c, D.tpos, cap)) ||
             (st = sim_alloc(c, D.sstart, nb + 1)) || (st = sim_alloc(c, D.tstart, nb)) ||
+            (st = sim_alloc(c, D.e_who, d.V)) || (st = sim_alloc(c, D.h_who, d.V)) ||
             (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
             (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
             (st = sim_alloc(c, D.torder, cap)) || (st = sim_alloc(c, D.tgrp, D.ngrp)) ||
@@ -834,7 +834,11 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
             return st;
         HIPCHK(c, hipMemset(D.bmask, 0, size_t(nb) * 32 * 4));
         HIPCHK(c, hipMemset(D.scnt, 0, size_t(nb) * 4));
-        HIPCHK(c, hipMemset(D.tcnt, 0, size_t(nb) * 4));
+        // a target slot that no carrier fills (two of one species on a site, a
+        // state the device flags) keeps a valid carrier index
+        HIPCHK(c, hipMemset(D.torder, 0, size_t(cap) * 4));
+        HIPCHK(c, hipMemset(D.e_who, 0, size_t(d.V) * 4));
+        HIPCHK(c, hipMemset(D.h_who, 0, size_t(d.V) * 4));
         {
             const std::vector<int> ones(D.ngrp, 1);   // no culled pass yet: equal shares
             HIPCHK(c, hipMemcpy(D.tgrp, ones.data(), ones.size() * 4, hipMemcpyHostToDevice));

@@ -70,9 +70,9 @@ struct SimDev {
     int cut, ngrp;
     int nbx, nby, nbz, nbrick;
     uint32_t* bmask;              // [brick][electrons, holes][16] site bits (cleared by decide_kernel)
-    int* scnt;                    // carriers per brick (cull_mask_kernel; cleared by decide_kernel)
-    int* tcnt;                    // targets per brick (same)
-    int2* tpos;                   // per target: its brick, its slot in the brick
+    int* scnt;                    // carriers per brick (cull_mark; cleared by decide_kernel)
+    int* e_who;                   // [V] list index of the electron on a site (valid where its bit is set)
+    int* h_who;                   // [V] the same for holes
     int* sstart;                  // [nbrick + 1] charge list offsets
     int* tstart;                  // [nbrick] first target slot of each brick (bricks in bperm order)
     const int* bperm;             // brick order of the target groups (static, set at load)
@@ -136,26 +136,17 @@ __device__ __forceinline__ void kahan(float& s, float& c, float x) {
     s = t;
 }
 
-// exclusive scan of one int per thread over a workgroup of nw waves
+// exclusive scan of one int per thread over a workgroup of nw waves (DPP
+// wave scans; wsum holds nw ints)
 template <int kThreads>
 __device__ int wg_scan(int v, int* wsum, int& total) {
     constexpr int nw = kThreads / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const int x = wave_incl_sum(v);
     if (lane == 63) wsum[wid] = x;
     __syncthreads();
     if (wid == 0) {
-        int w = lane < nw ? wsum[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int y = __shfl_up(w, o, 64);
-            if (lane >= o) w += y;
-        }
+        const int w = wave_incl_sum(lane < nw ? wsum[lane] : 0);
         if (lane < nw) wsum[lane] = w;
     }
     __syncthreads();
@@ -163,6 +154,33 @@ __device__ int wg_scan(int v, int* wsum, int& total) {
     total = wsum[nw - 1];
     __syncthreads();
     return excl;
+}
+
+// two exclusive scans at once (wsum holds 2 nw ints)
+template <int kThreads>
+__device__ void wg_scan2(int a, int b, int* wsum, int& ea, int& eb, int& ta, int& tb) {
+    constexpr int nw = kThreads / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int xa = wave_incl_sum(a), xb = wave_incl_sum(b);
+    if (lane == 63) {
+        wsum[wid] = xa;
+        wsum[nw + wid] = xb;
+    }
+    __syncthreads();
+    if (wid == 0) {
+        const int wa = wave_incl_sum(lane < nw ? wsum[lane] : 0);
+        const int wb = wave_incl_sum(lane < nw ? wsum[nw + lane] : 0);
+        if (lane < nw) {
+            wsum[lane] = wa;
+            wsum[nw + lane] = wb;
+        }
+    }
+    __syncthreads();
+    ea = xa - a + (wid > 0 ? wsum[wid - 1] : 0);
+    eb = xb - b + (wid > 0 ? wsum[nw + wid - 1] : 0);
+    ta = wsum[nw - 1];
+    tb = wsum[2 * nw - 1];
+    __syncthreads();
 }
 
 // Workgroup prefix over per-workgroup counts, sum of arr[stride * b + off]
@@ -175,10 +193,11 @@ __device__ __forceinline__ long long wg_prefix_loads(const int* arr, int nb, int
         for (int b = threadIdx.x; b < nb; b += 64) s += arr[stride * b + off];
     return s;
 }
+// (the counts are carriers: their sums fit an int)
 __device__ __forceinline__ long long wg_prefix_total(long long s, long long* slot) {
     if (threadIdx.x < 64) {
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (threadIdx.x == 0) *slot = s;
+        const int t = wave_total(wave_incl_sum(int(s)));
+        if (threadIdx.x == 0) *slot = t;
     }
     __syncthreads();
     return *slot;
@@ -409,13 +428,15 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 //
 //   cull_mark (tick_kernel for the kept carriers at their new sites,
 //     inject_kernel for the injected ones, or for all after a load): each
-//     carrier's bitmask bit, per-brick charge and target counts, and the
-//     carrier's slot among its brick's targets.
-//   inject_kernel: the counts become list offsets.
-//   cull_emit_kernel: one wave per brick writes its charge list; the other
-//     workgroups make the proposals, set the decision-draw flags and put the
-//     targets into their brick's slots; the last one writes cull_de_kernel's
-//     work items.
+//     carrier's bitmask bit, its brick's carrier count and its list index at
+//     its site.
+//   inject_kernel: the counts become list offsets (charges in linear brick
+//     order, targets in bperm order).
+//   cull_emit_kernel: one wave per brick writes its charge list and puts the
+//     brick's carriers (e_who/h_who of each set bit) into its target slots in
+//     the same order (electrons, then holes, each by site); the other
+//     workgroups make the proposals and set the decision-draw flags; the last
+//     one writes cull_de_kernel's work items.
 //   cull_de_kernel: the pair sums of every target over the lists of the bricks
 //     near its group (below), as int64 fixed-point totals.
 //   decide_kernel (committing pass) clears the bitmask words and brick
@@ -439,63 +460,71 @@ __device__ __forceinline__ int load_agent(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// a carrier at its current site into its brick: bitmask bit, charge count, and
-// its slot among the brick's targets, kept per list entry (electron li at
-// tpos[li], hole li at tpos[cap_e + li]: an injected electron does not move
-// the holes' entries)
+// a carrier (list index li) at its current site into its brick: bitmask bit,
+// carrier count (neither atomic returns a value) and the site's list index
 __device__ __forceinline__ void cull_mark(const SimDev& D, bool isE, int li, int site) {
     int x, y, z;
     decode(D.d, site, x, y, z);
     uint32_t bit;
     atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
-    const int b = cull_brick(D, x, y, z);
-    atomicAdd(&D.scnt[b], 1);
-    D.tpos[isE ? li : D.cap_e + li] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
+    atomicAdd(&D.scnt[cull_brick(D, x, y, z)], 1);
+    (isE ? D.e_who : D.h_who)[site] = li;
 }
 
 // Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the
 // targets (bricks in bperm order; tst[brick]) by one workgroup of kThreads;
-// thread t scans a contiguous range of bricks, its first kPer counts loaded
-// together.  Each brick's counts are the marks of the brick counters plus the
-// carriers listed in inj_b (bricks of carriers not yet counted there).  kAgent
-// reads the counters at agent scope (needed when this workgroup marked them).
+// thread t scans a contiguous range of bricks, its first kPer bricks' static
+// data loaded together.  A brick's carriers (its targets) are the marks of its
+// counter plus the carriers listed in inj_b (not yet counted there); its
+// charges add the static defects.  With cnt (LDS, nbrick <= kStageBricks) the
+// counters are read once, coalesced, and the injected carriers added there, so
+// that the bperm-ordered scan reads LDS.  kAgent reads the counters at agent
+// scope (needed when this workgroup marked them).
+constexpr int kStageBricks = 8192;
 template <int kThreads, bool kAgent>
-__device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, const int* inj_b, int inj_n) {
+__device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, const int* inj_b, int inj_n,
+                             int* cnt) {
     constexpr int kPer = 8;
     const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
     const int b0 = min(nb, int(threadIdx.x) * per), b1 = min(nb, b0 + per);
     auto ld = [&](const int* p) { return kAgent ? load_agent(p) : *p; };
-    auto injc = [&](int b) {
-        int n = 0;
-        for (int k = 0; k < inj_n; ++k) n += inj_b[k] == b;
-        return n;
-    };
-    auto scount = [&](int b) { return ld(&D.scnt[b]) + injc(b) + D.dstart[b + 1] - D.dstart[b]; };
-    auto tcount = [&](int b) { return ld(&D.tcnt[b]) + injc(b); };
-    // unconditional loads (clamped index) so that all are in flight together
-    int bp[kPer], sc[kPer], tc[kPer];
-    int srun = 0, trun = 0;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) bp[u] = D.bperm[min(b0 + u, nb - 1)];
+    // static data first (unconditional loads, clamped index), in flight with the counters
+    int bp[kPer], dn[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int b = min(b0 + u, nb - 1);
-        sc[u] = scount(b);
-        tc[u] = tcount(bp[u]);
+        bp[u] = D.bperm[b];
+        dn[u] = D.dstart[b + 1] - D.dstart[b];
     }
+    if (cnt) {
+        for (int b = threadIdx.x; b < nb; b += kThreads) cnt[b] = ld(&D.scnt[b]);
+        __syncthreads();
+        for (int k = threadIdx.x; k < inj_n; k += kThreads) atomicAdd(&cnt[inj_b[k]], 1);
+        __syncthreads();
+    }
+    auto carriers = [&](int b) {
+        if (cnt) return cnt[b];
+        int n = ld(&D.scnt[b]);
+        for (int k = 0; k < inj_n; ++k) n += inj_b[k] == b;
+        return n;
+    };
+    int sc[kPer], tc[kPer];
+    int srun = 0, trun = 0;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
+        sc[u] = carriers(min(b0 + u, nb - 1)) + dn[u];
+        tc[u] = carriers(bp[u]);
         if (b0 + u >= b1) sc[u] = tc[u] = 0;
         srun += sc[u];
         trun += tc[u];
     }
+    auto scount = [&](int b) { return carriers(b) + D.dstart[b + 1] - D.dstart[b]; };
     for (int b = b0 + kPer; b < b1; ++b) {
         srun += scount(b);
-        trun += tcount(D.bperm[b]);
+        trun += carriers(D.bperm[b]);
     }
-    int soff, toff;
-    int so = wg_scan<kThreads>(srun, wsum, soff);
-    int to = wg_scan<kThreads>(trun, wsum, toff);
+    int so, to, soff, toff;
+    wg_scan2<kThreads>(srun, trun, wsum, so, to, soff, toff);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int b = b0 + u;
@@ -511,7 +540,7 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, con
         sst[b] = so;
         tst[q] = to;
         so += scount(b);
-        to += tcount(q);
+        to += carriers(q);
     }
     if (threadIdx.x == 0) sst[nb] = soff;
 }
@@ -605,8 +634,6 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
             const uint8_t tf = isE ? D.etype[P.fs] : D.htype[P.fs];
             draw = (tf == kEmpty || tf == kDrain);
             D.flag[i] = draw;
-            const int2 tp = D.tpos[isE ? li : D.cap_e + li];
-            D.torder[tst[tp.x] + tp.y] = i;
         }
         const int cnt = __syncthreads_count(draw);
         if (threadIdx.x == 0) D.blk_draw[wx - nbw] = cnt;
@@ -617,39 +644,31 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
     const int lane = threadIdx.x & 63;
     const uint32_t v = lane < 32 ? D.bmask[b * 32 + lane] : 0u;
     const int cnt = __popc(v);
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const int n = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += n;
-    }
-    const int neh = __shfl(incl, 31, 64);
-    if (lane == 0 && neh != D.scnt[b]) set_error(D, kErrSite);   // two of one species on a site
-    const int base = sst[b], end = sst[b + 1];
+    const int incl = wave_incl_sum(cnt);
+    const int neh = __builtin_amdgcn_readlane(incl, 31);
+    const int nmark = D.scnt[b];
+    if (lane == 0 && neh != nmark) set_error(D, kErrSite);   // two of one species on a site
+    const int base = sst[b], end = sst[b + 1], t0 = tst[b], ne = D.ctl->n_e;
     const int bx = b % D.nbx, byz = b / D.nbx, by = byz % D.nby, bz = byz / D.nby;
-    const float q = lane < 16 ? -1.f : 1.f;
+    const bool isE = lane < 16;
+    const float q = isE ? -1.f : 1.f;
+    const int* who = isE ? D.e_who : D.h_who;
     int k = base + incl - cnt;
     for (uint32_t m = v; m && k < end; m &= m - 1u, ++k) {
         const int j = 32 * (lane & 15) + __builtin_ctz(m);
-        D.src[k] = make_float4(float(kBrick * bx + (j & 7)), float(kBrick * by + ((j >> 3) & 7)),
-                               float(kBrick * bz + (j >> 6)), q);
+        const int x = kBrick * bx + (j & 7), y = kBrick * by + ((j >> 3) & 7), z = kBrick * bz + (j >> 6);
+        D.src[k] = make_float4(float(x), float(y), float(z), q);
+        // the carrier on this site is the brick's target of the same rank
+        if (k - base < nmark) {
+            const int li = who[x + D.d.X * (y + D.d.Y * z)];
+            D.torder[t0 + (k - base)] = isE ? li : ne + li;
+        }
     }
     const int d0 = D.dstart[b], nd = D.dstart[b + 1] - d0;
     for (int u = lane; u < nd && base + neh + u < end; u += 64) D.src[base + neh + u] = D.dsrc[d0 + u];
     // slots left when two carriers of one species share a site (flagged above):
     // zero charges far away (their terms are exactly 0)
     for (int u = base + neh + nd + lane; u < end; u += 64) D.src[u] = make_float4(-1e4f, -1e4f, -1e4f, 0.f);
-}
-
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
 }
 
 // t / d and t % d for 0 <= t < 2^24 with inv = 1 / d (float estimate, corrected)
@@ -666,23 +685,8 @@ __device__ __forceinline__ int fdiv(int t, int d, float inv, int& r) {
     return q;
 }
 
-__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    return x - v;
-}
-
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 constexpr int kList = 512;   // LDS charge slots per wave
+constexpr int kGather = 8;   // units whose loads a wave keeps in flight together
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
 
@@ -902,29 +906,31 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
                     uends[wv][uo++] = ((k8[j] + fr[j] + k) << 7) | min(kUnit, td[j] - k);
             nu = wave_total(wave_incl_sum(munits));
             __builtin_amdgcn_wave_barrier();
-            // every lane copies one slot of a unit, two units at a time; the near
-            // charges of each unit are packed in order (ballot prefix)
+            // every lane copies one slot of a unit, kGather units at a time (their
+            // loads in flight together); the near charges of each unit are packed
+            // in order (ballot prefix)
             const v4f* src = reinterpret_cast<const v4f*>(D.src);
             v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]);
             int o = 0;
             const v4f far = {-1e4f, -1e4f, -1e4f, 0.f};
-            for (int u = 0; u < nu; u += 2) {
-                const int d0 = __builtin_amdgcn_readfirstlane(uends[wv][u]);
-                const int d1 = u + 1 < nu ? __builtin_amdgcn_readfirstlane(uends[wv][u + 1]) : 0;
-                const bool in0 = lane < (d0 & 127), in1 = lane < (d1 & 127);
-                const v4f a0 = in0 ? src[(d0 >> 7) + lane] : far;
-                const v4f a1 = in1 ? src[(d1 >> 7) + lane] : far;
-                const bool k0 = in0 && near(a0), k1 = in1 && near(a1);
-                const uint64_t m0 = __ballot(k0), m1 = __ballot(k1);
-                const int n0 = __popcll(m0);
-                if (k0) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m0 >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m0), 0))] = a0;
-                uends[wv][u] = (o << 16) | (o + n0);
-                o += n0;
-                if (u + 1 < nu) {
-                    const int n1 = __popcll(m1);
-                    if (k1) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m1 >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m1), 0))] = a1;
-                    uends[wv][u + 1] = (o << 16) | (o + n1);
-                    o += n1;
+            for (int u = 0; u < nu; u += kGather) {
+                v4f a[kGather];
+                bool in[kGather];
+#pragma unroll
+                for (int j = 0; j < kGather; ++j) {
+                    const int d = u + j < nu ? __builtin_amdgcn_readfirstlane(uends[wv][u + j]) : 0;
+                    in[j] = lane < (d & 127);
+                    a[j] = in[j] ? src[(d >> 7) + lane] : far;
+                }
+#pragma unroll
+                for (int j = 0; j < kGather; ++j) {
+                    if (u + j >= nu) break;
+                    const bool k = in[j] && near(a[j]);
+                    const uint64_t m = __ballot(k);
+                    if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = a[j];
+                    const int n = __popcll(m);
+                    uends[wv][u + j] = (o << 16) | (o + n);
+                    o += n;
                 }
             }
             fill = o;
@@ -971,9 +977,9 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
     const int nb = (N + kSimBlock - 1) / kSimBlock;
     if (blockIdx.x == 0 && threadIdx.x < 64 && !only_de) {
-        long long s = 0;
+        int s = 0;
         for (int b = threadIdx.x; b < nb; b += 64) s += D.blk_draw[b];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        s = wave_total(wave_incl_sum(s));
         if (threadIdx.x == 0) {
             D.ctl->draws_dec = (unsigned long long)s;
             D.ctl->draws_rec = 0ull;
@@ -1007,9 +1013,7 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
             decode(D.d, site, x, y, z);
             uint32_t bit;
             D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)] = 0u;
-            const int b = cull_brick(D, x, y, z);
-            D.scnt[b] = 0;
-            D.tcnt[b] = 0;
+            D.scnt[cull_brick(D, x, y, z)] = 0;
         }
     }
     double de = 0.0;
@@ -1135,7 +1139,7 @@ struct Serial {
 constexpr int kInjMax = 2 * 1025 + 8;
 
 struct OneShared {
-    int wsum[16];
+    int wsum[32];
     unsigned long long pos;
     int n_e, n_h, cur;
     int first_bad;
@@ -1144,9 +1148,10 @@ struct OneShared {
     unsigned int att[10], suc[10];
     double red[16];
     uint32_t tail[1024];
-    // carriers injected this step on the culled path: brick, and tpos index
+    // carriers injected this step on the culled path: their bricks
     int inj_n;
-    int inj_b[kInjMax], inj_t[kInjMax];
+    int inj_b[kInjMax];
+    int cnt[kStageBricks];   // inject_kernel's list offsets: per-brick carriers
 };
 
 __device__ double wg_sum_d(double v, double* red) {
@@ -1230,21 +1235,19 @@ __global__ __launch_bounds__(kOneBlock) void recomb_kernel(const SimDev* __restr
 // ===========================================================================
 __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];   // replica
-    __shared__ int wsum[4];
+    __shared__ int wsum[8];
     __shared__ long long pre_e, pre_h;
     const SimCtl& C = *D.ctl;
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur, nxt = cur ^ 1;
     const int nb = (N + kSimBlock - 1) / kSimBlock;
     if (blockIdx.x == 0 && threadIdx.x < 64) {
-        long long se = 0, sh_ = 0;
+        int se = 0, sh_ = 0;
         for (int b = threadIdx.x; b < nb; b += 64) {
             se += D.blk_keep[2 * b];
             sh_ += D.blk_keep[2 * b + 1];
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            se += __shfl_down(se, o, 64);
-            sh_ += __shfl_down(sh_, o, 64);
-        }
+        se = wave_total(wave_incl_sum(se));
+        sh_ = wave_total(wave_incl_sum(sh_));
         if (threadIdx.x == 0) {
             D.ctl->n_e_next = int(se);
             D.ctl->n_h_next = int(sh_);
@@ -1286,9 +1289,8 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
     }
     const long long base_e = wg_prefix_total(keep_e_lane, &pre_e);
     const long long base_h = wg_prefix_total(keep_h_lane, &pre_h);
-    int te, th;
-    const int oe = wg_scan<kSimBlock>(keep_e, wsum, te);
-    const int oh = wg_scan<kSimBlock>(keep_h, wsum, th);
+    int oe, oh, te, th;
+    wg_scan2<kSimBlock>(keep_e, keep_h, wsum, oe, oh, te, th);
     if (keep_e) {
         const int w = int(base_e) + oe;
         D.e_site[nxt][w] = dest;
@@ -1307,21 +1309,21 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
 // ===========================================================================
 // Injection (simulation.cpp:233-317, sourceagent.cpp:92-247) and proposals.
 // ===========================================================================
-// an injected carrier on the culled path: its bitmask bit now; its brick goes
-// into the list that inject_kernel's offsets add and then turns into counts and
-// a target slot (cull_mark's other half)
+// an injected carrier (list index li) on the culled path: its bitmask bit and
+// the site's list index now; its brick goes into the list that inject_kernel's
+// offsets add and then into the brick's counter (cull_mark's other half)
 __device__ void inject_mark(const SimDev& D, OneShared& sh, bool isE, int li, int site) {
     int x, y, z;
     decode(D.d, site, x, y, z);
     uint32_t bit;
     atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
+    (isE ? D.e_who : D.h_who)[site] = li;
     const int k = sh.inj_n;
     if (k >= kInjMax) {
         set_error(D, kErrCapacity);
         return;
     }
     sh.inj_b[k] = cull_brick(D, x, y, z);
-    sh.inj_t[k] = isE ? li : D.cap_e + li;
     sh.inj_n = k + 1;
 }
 
@@ -1598,15 +1600,12 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
                 cull_mark(D, isE, li, isE ? D.e_site[cur][li] : D.h_site[cur][li]);
             }
             __syncthreads();
-            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0);
+            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0,
+                                          D.nbrick <= kStageBricks ? sh.cnt : nullptr);
         } else {   // tick_kernel's marks plus this step's injected carriers
-            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n);
-            __syncthreads();
-            for (int k = threadIdx.x; k < sh.inj_n; k += blockDim.x) {
-                const int b = sh.inj_b[k];
-                atomicAdd(&D.scnt[b], 1);
-                D.tpos[sh.inj_t[k]] = make_int2(b, atomicAdd(&D.tcnt[b], 1));
-            }
+            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n,
+                                           D.nbrick <= kStageBricks ? sh.cnt : nullptr);
+            for (int k = threadIdx.x; k < sh.inj_n; k += blockDim.x) atomicAdd(&D.scnt[sh.inj_b[k]], 1);
         }
     }
     __syncthreads();
