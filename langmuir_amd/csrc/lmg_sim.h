@@ -473,32 +473,45 @@ __device__ __forceinline__ void cull_mark(const SimDev& D, bool isE, int li, int
 
 // Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the
 // targets (bricks in bperm order; tst[brick]) by one workgroup of kThreads;
-// thread t scans a contiguous range of bricks, its first kPer bricks' static
-// data loaded together.  A brick's carriers (its targets) are the marks of its
+// thread t scans a contiguous range of bricks, its first kOffPer bricks' static
+// data loaded together (cull_offsets_load).  A brick's carriers (its targets) are the marks of its
 // counter plus the carriers listed in inj_b (not yet counted there); its
 // charges add the static defects.  With cnt (LDS, nbrick <= kStageBricks) the
 // counters are read once, coalesced, and the injected carriers added there, so
 // that the bperm-ordered scan reads LDS.  kAgent reads the counters at agent
 // scope (needed when this workgroup marked them).
 constexpr int kStageBricks = 8192;
+constexpr int kOffPer = 8;
+// The loads of the offsets, issued before the injections (no barrier): the
+// thread's first kOffPer bricks' bperm entries and defect counts, and with cnt
+// the counters into LDS.
+struct OffsetLoads {
+    int bp[kOffPer], dn[kOffPer];
+};
+template <int kThreads, bool kAgent>
+__device__ void cull_offsets_load(const SimDev& D, int* cnt, OffsetLoads& L) {
+    const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
+    const int b0 = min(nb, int(threadIdx.x) * per);
+#pragma unroll
+    for (int u = 0; u < kOffPer; ++u) {   // unconditional (clamped index): all in flight together
+        const int b = min(b0 + u, nb - 1);
+        L.bp[u] = D.bperm[b];
+        L.dn[u] = D.dstart[b + 1] - D.dstart[b];
+    }
+    if (cnt)
+        for (int b = threadIdx.x; b < nb; b += kThreads) cnt[b] = kAgent ? load_agent(&D.scnt[b]) : D.scnt[b];
+}
 template <int kThreads, bool kAgent>
 __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, const int* inj_b, int inj_n,
-                             int* cnt) {
-    constexpr int kPer = 8;
+                             int* cnt, const OffsetLoads& L) {
+    constexpr int kPer = kOffPer;
     const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
     const int b0 = min(nb, int(threadIdx.x) * per), b1 = min(nb, b0 + per);
     auto ld = [&](const int* p) { return kAgent ? load_agent(p) : *p; };
-    // static data first (unconditional loads, clamped index), in flight with the counters
-    int bp[kPer], dn[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int b = min(b0 + u, nb - 1);
-        bp[u] = D.bperm[b];
-        dn[u] = D.dstart[b + 1] - D.dstart[b];
-    }
+    const int* bp = L.bp;
+    const int* dn = L.dn;
     if (cnt) {
-        for (int b = threadIdx.x; b < nb; b += kThreads) cnt[b] = ld(&D.scnt[b]);
-        __syncthreads();
+        __syncthreads();   // the staged counters
         for (int k = threadIdx.x; k < inj_n; k += kThreads) atomicAdd(&cnt[inj_b[k]], 1);
         __syncthreads();
     }
@@ -973,6 +986,7 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     __shared__ int wsum[4];
     __shared__ long long pre;
     __shared__ unsigned int att[4], suc[4];
+    __shared__ uint32_t win[kSimBlock];   // the workgroup's decision words
     const SimCtl& C = *D.ctl;
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur;
     const int nb = (N + kSimBlock - 1) / kSimBlock;
@@ -1016,6 +1030,10 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
             D.scnt[cull_brick(D, x, y, z)] = 0;
         }
     }
+    // the draws before this workgroup (a barrier: every load above is in), then
+    // the engine outputs its draws can take, fetched while dE is assembled
+    const long long before = only_de ? 0 : wg_prefix_total(draws_lane, &pre);
+    const uint32_t word = only_de ? 0u : rng_word(D, C.rng_pos + (unsigned long long)(before + threadIdx.x));
     double de = 0.0;
     if (valid && D.coulomb) {
         double sde;
@@ -1048,13 +1066,13 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     if (only_de) return;
 
     if (threadIdx.x < 4) { att[threadIdx.x] = 0; suc[threadIdx.x] = 0; }
-    const long long before = wg_prefix_total(draws_lane, &pre);
+    win[threadIdx.x] = word;
     int total;
-    const int off = wg_scan<kSimBlock>(draw, wsum, total);
+    const int off = wg_scan<kSimBlock>(draw, wsum, total);   // (its barriers publish win)
     int code = 0, keep = 1;
     if (valid) {
         if (draw) {
-            const double u = u01(rng_word(D, C.rng_pos + (unsigned long long)(before + off)));
+            const double u = u01(win[off]);   // engine output rng_pos + before + off
             if (tf == kEmpty) {
                 double pd = vb_fs - vb_site;
                 pd *= q;
@@ -1542,6 +1560,20 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
         sh.first_bad = 0x7fffffff;
         sh.inj_n = 0;
     }
+    // after a step: the offsets' loads and the listed engine outputs before the
+    // injections (neither changes there)
+    int* cnt = D.cull && D.nbrick <= kStageBricks ? sh.cnt : nullptr;
+    OffsetLoads ol;
+    int nd = 0;
+    unsigned long long dl[kDangerSlots];
+    if (mode == 0) {
+        if (D.cull) cull_offsets_load<kOneBlock, false>(D, cnt, ol);
+        if (threadIdx.x == 0) {
+            nd = C.danger_n;
+#pragma unroll
+            for (int k = 0; k < kDangerSlots; ++k) dl[k] = C.danger[k];
+        }
+    }
     __syncthreads();
     if (mode == 1) mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
     else inject_all(D, sh);
@@ -1552,10 +1584,14 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
     // outputs that mt_refill listed can be rejected: thread 0 checks those in range.
     // Should the list have wrapped past one of them, every carrier is checked.
     if (D.uniform && N > 0 && threadIdx.x == 0) {
-        const int nd = C.danger_n;
+        if (mode == 1) {
+            nd = C.danger_n;
+#pragma unroll
+            for (int k = 0; k < kDangerSlots; ++k) dl[k] = C.danger[k];
+        }
         unsigned long long oldest = ~0ull;
         for (int k = 0; k < min(nd, kDangerSlots); ++k) {
-            const unsigned long long p = C.danger[k];
+            const unsigned long long p = dl[k];
             oldest = p < oldest ? p : oldest;
             if (p < step0 || p >= step0 + (unsigned long long)N) continue;
             const int i = int(p - step0);
@@ -1600,11 +1636,10 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
                 cull_mark(D, isE, li, isE ? D.e_site[cur][li] : D.h_site[cur][li]);
             }
             __syncthreads();
-            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0,
-                                          D.nbrick <= kStageBricks ? sh.cnt : nullptr);
+            cull_offsets_load<kOneBlock, true>(D, cnt, ol);
+            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0, cnt, ol);
         } else {   // tick_kernel's marks plus this step's injected carriers
-            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n,
-                                           D.nbrick <= kStageBricks ? sh.cnt : nullptr);
+            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n, cnt, ol);
             for (int k = threadIdx.x; k < sh.inj_n; k += blockDim.x) atomicAdd(&D.scnt[sh.inj_b[k]], 1);
         }
     }
