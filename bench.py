@@ -285,7 +285,7 @@ def batch_workload(n_rep=256, seed=4):
     return reps
 
 
-def bench_batch(steps=200, warmup=20, n_rep=256, de_path="auto"):
+def bench_batch(steps=200, warmup=20, n_rep=256, de_path="auto", graphs=True):
     """256 independent replicas stepped together.  value: replica-steps/s over `steps` graph
     replays after a fixed-time warm-up; roofline of the replica-wide dE kernel from a
     kernel-by-kernel run with events around it and exact in-cutoff term counts of every
@@ -294,10 +294,12 @@ def bench_batch(steps=200, warmup=20, n_rep=256, de_path="auto"):
     reps = batch_workload(n_rep)
     t0 = time.perf_counter()
     batch = L.Batch(reps, de_path=de_path)
+    batch.set_graphs(graphs)
     log(f"batch of {n_rep} replicas created in {time.perf_counter() - t0:.1f} s")
     # The replicas lose carriers through their drains as they run, so the clocks are warmed
     # on a second copy of the batch and the timed one starts from the configured state.
     heater = L.Batch(reps, de_path=de_path)
+    heater.set_graphs(graphs)
     warm_until(heater.step, heater.sync, WARM_S, 20)
     heater.close()
     batch.step(warmup)
@@ -324,7 +326,7 @@ def bench_batch(steps=200, warmup=20, n_rep=256, de_path="auto"):
                event_ms_per_step=round(ms / steps, 4),
                replica_steps_per_s=round(n_rep * steps / wall, 1),
                pair_evals_per_s=pair_evals * steps / wall,
-               carriers_total=int(sum(ns)), graph_replay=True, autotuned=batch.tuning,
+               carriers_total=int(sum(ns)), graph_replay=graphs, autotuned=batch.tuning,
                culled_replicas=paths.count("culled"),
                kernel_us={"de_phase": round(de_us, 2), "rest_of_step": round(rest_ms * 1e3 / n_ph, 2),
                           "event_span_per_step": round(span_ms * 1e3 / n_ph, 2),
@@ -728,7 +730,7 @@ def main():
     if not args.no_map:
         res["map"], map_samples = bench_map()
     if not args.no_batch:
-        res["batch"] = bench_batch(de_path=args.batch_de_path)
+        res["batch"] = bench_batch(de_path=args.batch_de_path, graphs=not args.eager)
     if not args.no_surrogate and args.workload == "solar":
         res["surrogate"] = bench_surrogate(workload, n_snap=max(16, args.surrogate_snapshots),
                                            iters=args.surrogate_iters)

@@ -718,8 +718,9 @@ constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's lis
 //     LDS list (its own 64-byte rows, all lanes at once) at the exclusive scan
 //     of the counts; the list is summed whenever the next bricks do not fit;
 //   sum: every lane runs over the list (broadcast reads): an FP32 sum per unit,
-//     taken to fixed point (x 2^40, truncated) and added in int64; the chunk's
-//     total goes into the target's isum with a 64-bit integer atomic.
+//     taken to fixed point (x 2^40, truncated) and added in int64; the four
+//     chunks' totals are added in LDS and go into the target's isum with one
+//     64-bit integer atomic.
 // The units and their FP32 sums are the same for a target in any group,
 // and the units a group skips hold only terms that are exactly 0 for its
 // targets; integer addition is exact, so each target's total is the same
@@ -973,7 +974,16 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         }
     }
     flush();
-    if (valid && acc64 != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&D.isum[i]), (unsigned long long)acc64);
+    // the workgroup's four chunks belong to the same 64 targets: their totals are
+    // added in LDS (exact int64) and each target takes one atomic
+    __syncthreads();   // every wave is done with its list
+    long long* part = reinterpret_cast<long long*>(&lists[0][0]);
+    part[threadIdx.x] = acc64;
+    __syncthreads();
+    if (wv == 0) {
+        const long long t = (part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]);
+        if (valid && t != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&D.isum[i]), (unsigned long long)t);
+    }
 }
 
 // ===========================================================================
