@@ -579,7 +579,7 @@ def maps_vs_fp64(samples, threads):
 
 
 # ---------------------------------------------------------------------------
-def step_leg(workload, steps, warmup, de_path, breakdown_steps, graphs=True):
+def step_leg(workload, steps, warmup, de_path, breakdown_steps, graphs=True, de_shape=0):
     """One simulation: `warmup` steps plus WARM_S seconds of untimed steps, then `steps`
     timed graph replays between device synchronisations; then a kernel-by-kernel run of
     `breakdown_steps` steps with HIP events for the phase and pair-sum-kernel times, with
@@ -587,7 +587,7 @@ def step_leg(workload, steps, warmup, de_path, breakdown_steps, graphs=True):
     import langmuir_amd as L
     p, e, h, traps, pot = workload
     dims = (p.grid_x, p.grid_y, p.grid_z)
-    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=de_path)
+    sim = L.Simulation(p, electrons=e, holes=h, potential=pot, de_path=de_path, de_chunks=de_shape)
     sim.set_graphs(graphs)
     sim.step(warmup)
     sim.ctx.sync()
@@ -647,6 +647,9 @@ def main():
     ap.add_argument("--de-path", choices=("auto", "pairs", "culled"), default="auto",
                     help="dE path of the stepped simulation (auto: culled when it pays)")
     ap.add_argument("--breakdown-steps", type=int, default=1000)
+    ap.add_argument("--de-shape", type=int, default=0, metavar="N",
+                    help="fix the main leg's dE kernel shape instead of autotuning it: charge chunks (all "
+                         "pairs) or workgroups (culled); for counter runs, whose timings mislead the tuner")
     ap.add_argument("--eager", action="store_true",
                     help="launch the steps kernel by kernel instead of replaying a captured HIP graph "
                          "(for profilers that do not follow graph launches)")
@@ -685,7 +688,7 @@ def main():
     log(f"workload {args.workload}: {p.grid_x}x{p.grid_y}x{p.grid_z}, {len(workload[1])} e + {len(workload[2])} h, "
         f"cutoff {p.electrostatic_cutoff}")
     main_leg = step_leg(workload, args.steps, args.warmup, args.de_path, max(args.breakdown_steps, 1),
-                        graphs=not args.eager)
+                        graphs=not args.eager, de_shape=args.de_shape)
     k = main_leg["kernel_us"]
     kname = main_leg["roofline"]["kernel"]
     log(f"{main_leg['steps_per_s']:.1f} steps/s ({main_leg['ms_per_step'] * 1e3:.1f} us/step, graph replay, after "
