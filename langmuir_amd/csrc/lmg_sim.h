@@ -737,6 +737,12 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
+    // Every wave starts with a chain of dependent loads (its box, candidate
+    // bricks and list units) while other waves of its SIMD may already be
+    // summing: the list building runs at a raised issue priority, the sums at
+    // the normal one, so that a wave's latency-bound part is not held back by
+    // the others' arithmetic.
+    __builtin_amdgcn_s_setprio(1);
     const int nt = D.ctl->counts[0];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (wx >= *D.nitems) return;   // whole workgroup
@@ -799,6 +805,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     long long acc64 = 0;
     int fill = 0, nu = 0;   // list slots and units in use
     auto flush = [&]() {
+        __builtin_amdgcn_s_setprio(0);
         __builtin_amdgcn_wave_barrier();
         for (int q = 0; q < nu; ++q) {
             const int se = __builtin_amdgcn_readfirstlane(uends[wv][q]);
@@ -817,6 +824,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             acc64 += (long long)(acc * kFix);
         }
         __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_setprio(1);
         fill = 0;
         nu = 0;
     };
