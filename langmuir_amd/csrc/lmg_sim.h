@@ -887,8 +887,9 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
 
     constexpr int kPer = 8;   // candidates per lane kept in registers
     if (ncand <= 64 * kPer) {   // the same for the four waves (one group)
-        int* tab_n = &uends[0][0];   // 512 candidates' list lengths and starts, before
-        int* tab_k = &uends[1][0];   // the unit ends take this space
+        int* tab_n = &uends[0][0];   // 512 candidates' list lengths and starts (the first
+        int* tab_k = tab_n + 64 * kPer;   // 1,024 ints of the unit tables, before the unit ends take them)
+        static_assert(2 * 64 * kPer <= 4 * kList, "candidate table in the unit tables");
         for (int ci = threadIdx.x; ci < 64 * kPer; ci += kSimBlock) {
             int k0;
             tab_n[ci] = brick(ci, k0);
