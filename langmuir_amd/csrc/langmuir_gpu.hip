@@ -1164,6 +1164,10 @@ lmg_status lmg_sim_delta_e(lmg_ctx* c, int32_t* fsite_e, int32_t* fsite_h, doubl
 struct lmg_batch {
     int dev = 0;
     hipStream_t stream = nullptr;
+    // the culled replicas' dE kernels run on a second stream, forked from and
+    // joined to the batch stream, beside the all-pairs replicas' de_kernel
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     std::vector<lmg_ctx*> reps;
     SimDev* d_sims = nullptr;   // every replica (the commit kernels)
@@ -1206,18 +1210,25 @@ lmg_status bfail(lmg_batch* b, lmg_status code, const std::string& msg) {
     } while (0)
 
 // the dE phase of every replica (pair sums, decision flags, random-stream
-// refill): the culled replicas' list building and sums, then de_kernel over the
-// all-pairs ones
+// refill): the culled replicas' list building and sums and de_kernel over the
+// all-pairs ones, concurrently (the culled ones on the side stream) when the
+// batch has both
 void batch_launch_de(lmg_batch* b) {
     const dim3 blk(kSimBlock);
     if (b->n_cull) {
-        hipLaunchKernelGGL(cull_emit_kernel, dim3(b->emit_wgs), blk, 0, b->stream, b->d_cull,
+        const bool both = b->n_pairs > 0;
+        hipStream_t st = both ? b->side : b->stream;
+        if (both) {
+            (void)hipEventRecord(b->fork, b->stream);
+            (void)hipStreamWaitEvent(b->side, b->fork, 0);
+        }
+        hipLaunchKernelGGL(cull_emit_kernel, dim3(b->emit_wgs), blk, 0, st, b->d_cull,
                            (const int2*)b->d_emit_map);
         if (b->gauss)
-            hipLaunchKernelGGL((cull_de_kernel<true>), dim3(b->de_wgs), blk, 0, b->stream, b->d_cull,
+            hipLaunchKernelGGL((cull_de_kernel<true>), dim3(b->de_wgs), blk, 0, st, b->d_cull,
                                (const int2*)b->d_de_map);
         else
-            hipLaunchKernelGGL((cull_de_kernel<false>), dim3(b->de_wgs), blk, 0, b->stream, b->d_cull,
+            hipLaunchKernelGGL((cull_de_kernel<false>), dim3(b->de_wgs), blk, 0, st, b->d_cull,
                                (const int2*)b->d_de_map);
     }
     if (b->n_pairs) {
@@ -1225,6 +1236,10 @@ void batch_launch_de(lmg_batch* b) {
         if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_pairs);
         else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_pairs);
         else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_pairs);
+        if (b->n_cull) {   // join the side stream
+            (void)hipEventRecord(b->join, b->side);
+            (void)hipStreamWaitEvent(b->stream, b->join, 0);
+        }
     }
 }
 
@@ -1344,7 +1359,8 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
 }
 
 // The batch's autotuner.  Candidates, each timed twice (the second round sees
-// the clocks the first one raised) and scored by its faster time:
+// the clocks the first one raised) and scored by its faster time (the chosen
+// split is then timed as it runs, its two parts concurrently):
 //   culled replicas: the total number of cull_de_kernel workgroups (one per
 //     target group at least, then 1 .. 4 device-resident waves of workgroups,
 //     or 1.5 and 2 per target group), shared out in proportion to each
@@ -1466,6 +1482,14 @@ lmg_status batch_plan(lmg_batch* b, int de_path) {
         const double t = tune_s(plan, "pairs_candidates", S_split);
         if (t < 0) return b->err.empty() ? bfail(b, LMG_ERR_HIP, "batch autotune failed") : LMG_ERR_HIP;
         split_us += t;
+        if (any_cull && de_path != 1) {   // both parts as they run, concurrently
+            const lmg_status st = batch_apply(b, plan, S_split, quads);
+            if (st != LMG_OK) return st;
+            double best = 1e30;
+            for (int round = 0; round < 2; ++round) timed(best);
+            js << "\"split_us\": " << best << ", ";
+            split_us = best;
+        }
     }
     int S = S_split;
     if (de_path == 0 && any_cull) {   // the same replicas all on the all-pairs path
@@ -1504,6 +1528,9 @@ void lmg_batch_destroy(lmg_batch* b) {
     for (void* p : {(void*)b->d_sims, (void*)b->d_pairs, (void*)b->d_cull, (void*)b->d_emit_map, (void*)b->d_de_map})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1548,7 +1575,10 @@ lmg_status lmg_batch_create_with_path(int32_t n, const lmg_params* params, const
             return give_up(LMG_ERR_ARG, "replicas must agree on coulomb.carriers and Gaussian screening");
         }
     }
-    if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
         return give_up(LMG_ERR_HIP, "batch stream creation failed");
     const lmg_status st = batch_plan(b, de_path);
     if (st != LMG_OK) {
