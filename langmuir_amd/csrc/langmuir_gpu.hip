@@ -513,6 +513,9 @@ lmg_status lmg_create(const lmg_params* params, lmg_ctx** out) {
     if (p.cutoff < 1 || p.cutoff > 16384) return fail(nullptr, LMG_ERR_ARG, "cutoff must be in [1, 16384]");
     if (p.hopping_range < 1 || p.hopping_range > 2)
         return fail(nullptr, LMG_ERR_ARG, "hopping_range must be 1 or 2");
+    // the dE kernels' fixed-point block sums are exact for |q| <= 31 e
+    if (p.defects_charge < -31 || p.defects_charge > 31)
+        return fail(nullptr, LMG_ERR_ARG, "defects_charge must be in [-31, 31]");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
         return fail(nullptr, LMG_ERR_NO_DEVICE, "no HIP device available");

@@ -236,5 +236,21 @@ __device__ __forceinline__ float wave_fmax(float v) {
                            [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); });
     return __int_as_float(__builtin_amdgcn_readlane(r, 63));
 }
+// integer minimum and maximum over the wave (one DPP min/max per step)
+__device__ __forceinline__ int wave_imin(int v) {
+    return __builtin_amdgcn_readlane(dpp_scan(v, 0x7fffffff, [](int a, int b) { return min(a, b); }), 63);
+}
+__device__ __forceinline__ int wave_imax(int v) {
+    return __builtin_amdgcn_readlane(dpp_scan(v, int(0x80000000), [](int a, int b) { return max(a, b); }), 63);
+}
+
+// A pointer read out of a device struct is a generic (flat) address; flat loads
+// count against the LDS/scalar counter as well, so any LDS or scalar wait also
+// waits for them.  Loads through gmem() are global loads, tracked by the vector
+// memory counter alone.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
 
 }  // namespace lmg

@@ -336,19 +336,31 @@ __device__ __forceinline__ float de_term(const float4& p1, const float4& q, cons
         const float fb = erff(gk * (r2 * b)) * b;
         return (in2 ? fb : 0.f) - (in1 ? fa : 0.f);
     }
-    // b - a = (r1 - r2) / (sqrt(r1) sqrt(r2) (sqrt(r1) + sqrt(r2))) = -t / (r1 r2 (a + b))
+    // b - a = (r1 - r2) / (sqrt(r1) sqrt(r2) (sqrt(r1) + sqrt(r2))) = -t / (r1 r2 (a + b));
+    // one term in cutoff: b or -a (three selects, no subtraction)
     const float both = -t * __builtin_amdgcn_rcpf((r1 * r2) * (a + b));
-    const float one = (in2 ? b : 0.f) - (in1 ? a : 0.f);
+    const float one = in2 ? b : (in1 ? -a : 0.f);
     return (in1 & in2) ? both : one;
 }
 
 // Fixed point of the pair sums: every FP32 sum of a summation unit (a block of
 // 32 charges here, a 64-slot piece of a brick list in cull_de_kernel) is
-// scaled by 2^40, truncated to int64 and added exactly, so a carrier's total
+// scaled by 2^40, rounded to an integer and added exactly, so a carrier's total
 // does not depend on how the units are split over workgroups or in what order
-// they are added (at most 2^-40 per unit, about 1e-10 eV per carrier).
-constexpr float kFix = 1099511627776.f;        // 2^40
-constexpr double kFixInv = 1.0 / 1099511627776.0;
+// they are added (at most 2^-41 per unit, about 1e-10 eV per carrier).
+//
+// The rounding is one FP64 fma: acc 2^40 + 1.5 2^52 lies in [2^52, 2^53) for
+// |acc| < 2^11, where the double's spacing is 1, so its bit pattern is
+// 0x4338000000000000 + round(acc 2^40).  A unit holds at most 64 terms of
+// magnitude <= |q| (both distances >= 1), so |acc| < 2^11 for charges up to
+// 31 e (lmg_create refuses larger defect charges).
+constexpr double kFixScale = 1099511627776.0;   // 2^40
+constexpr double kFixInv = 1.0 / kFixScale;
+constexpr double kFixMagic = 6755399441055744.0;   // 1.5 2^52
+__device__ __forceinline__ long long fix_round(float acc) {
+    const unsigned long long b = __double_as_longlong(fma(double(acc), kFixScale, kFixMagic));
+    return (long long)(b - 0x4338000000000000ull);
+}
 
 template <bool kGauss, bool kPairs>
 __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
@@ -402,13 +414,13 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 #pragma unroll 8
             for (int k = k0; k < k0 + 32; ++k)
                 acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
-            acc64 += (long long)(acc * kFix);
+            acc64 += fix_round(acc);
         }
         if (k0 < n) {   // the list's last, partial block
             float acc = 0.f;
             for (int k = k0; k < n; ++k)
                 acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
-            acc64 += (long long)(acc * kFix);
+            acc64 += fix_round(acc);
         }
     }
     if (i < nt) D.part[blockIdx.y * D.ld + i] = acc64;
@@ -606,6 +618,9 @@ __device__ void cull_items(const SimDev& D, int* wsum) {
         for (int q = 0; q < qg; ++q) D.items[k + q] = make_int4(g, q, qg, 0);
         k += qg;
     }
+    // item slots past the last (none while items_cap / 4 >= the group count)
+    // tell their workgroups to exit
+    for (int q = total + int(threadIdx.x); q < D.items_cap / 4; q += kSimBlock) D.items[q] = make_int4(-1, 0, 1, 0);
     if (threadIdx.x == 0) *D.nitems = total;
 }
 
@@ -743,37 +758,43 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     // the normal one, so that a wave's latency-bound part is not held back by
     // the others' arithmetic.
     __builtin_amdgcn_s_setprio(1);
-    const int nt = D.ctl->counts[0];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (wx >= *D.nitems) return;   // whole workgroup
-    const int4 it = D.items[wx];
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    // the target count and the item are independent loads
+    const int nt = *gmem(&D.ctl->counts[0]);
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i it = gmem(reinterpret_cast<const v4i*>(D.items))[wx];
+    if (it.x < 0 || nt <= 0) return;   // whole workgroup: no item (testing both issues both loads)
     const int g = it.x, s = 4 * it.y + wv, S = 4 * it.z;
     const int r = g * kCullGroup + lane;
     const bool valid = r < nt;
-    const int i = valid ? D.torder[r] : 0;
-    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f), p2 = p1;
-    if (valid) {
-        p1 = D.tgt1[i];
-        p2 = D.tgt2[i];
-    }
-    constexpr float kBig = 1e9f;
-    // the box is wave-uniform (scalar registers)
-    const int x0 = int(wave_fmin(valid ? fminf(p1.x, p2.x) : kBig));
-    const int y0 = int(wave_fmin(valid ? fminf(p1.y, p2.y) : kBig));
-    const int z0 = int(wave_fmin(valid ? fminf(p1.z, p2.z) : kBig));
-    const int x1 = int(wave_fmax(valid ? fmaxf(p1.x, p2.x) : -kBig));
-    const int y1 = int(wave_fmax(valid ? fmaxf(p1.y, p2.y) : -kBig));
-    const int z1 = int(wave_fmax(valid ? fmaxf(p1.z, p2.z) : -kBig));
+    // a group holds at least one target: the lanes past the last load its sites
+    // (no branch between the loads of the chain)
+    const int i = gmem(D.torder)[min(r, nt - 1)];
+    const v4f ta = gmem(reinterpret_cast<const v4f*>(D.tgt1))[i];
+    const v4f tb = gmem(reinterpret_cast<const v4f*>(D.tgt2))[i];
+    const float4 p1 = make_float4(ta.x, ta.y, ta.z, ta.w), p2 = make_float4(tb.x, tb.y, tb.z, tb.w);
+    // the box is wave-uniform (scalar registers); the coordinates are
+    // non-negative, so their float bit patterns order as integers
+    auto lo_of = [&](float a, float b) { return valid ? __float_as_int(fminf(a, b)) : 0x7fffffff; };
+    auto hi_of = [&](float a, float b) { return valid ? __float_as_int(fmaxf(a, b)) : int(0x80000000); };
+    const int x0 = int(__int_as_float(wave_imin(lo_of(p1.x, p2.x))));
+    const int y0 = int(__int_as_float(wave_imin(lo_of(p1.y, p2.y))));
+    const int z0 = int(__int_as_float(wave_imin(lo_of(p1.z, p2.z))));
+    const int x1 = int(__int_as_float(wave_imax(hi_of(p1.x, p2.x))));
+    const int y1 = int(__int_as_float(wave_imax(hi_of(p1.y, p2.y))));
+    const int z1 = int(__int_as_float(wave_imax(hi_of(p1.z, p2.z))));
     const int c = D.cut, c2 = c * c;
     const int bx0 = max(0, x0 - c + 1) >> 3, bx1 = min(D.d.X - 1, x1 + c - 1) >> 3;
     const int by0 = max(0, y0 - c + 1) >> 3, by1 = min(D.d.Y - 1, y1 + c - 1) >> 3;
     const int bz0 = max(0, z0 - c + 1) >> 3, bz1 = min(D.d.Z - 1, z1 + c - 1) >> 3;
     const int nx = bx1 - bx0 + 1, ny = by1 - by0 + 1, ncand = nx * ny * (bz1 - bz0 + 1);
-    const float inx = 1.f / float(nx), iny = 1.f / float(ny);
+    // estimates for fdiv, which corrects a quotient that is off by one
+    const float inx = __builtin_amdgcn_rcpf(float(nx)), iny = __builtin_amdgcn_rcpf(float(ny));
     const HopTerm ht = hop_term(p1, p2);
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
-    const int* __restrict__ sstart = D.sstart;
+    const __attribute__((address_space(1))) int* sstart = gmem(D.sstart);
     // candidate ci -> its list [k0, k0 + n) when kept, else n = 0
     auto brick = [&](int ci, int& k0) -> int {
         k0 = 0;
@@ -801,7 +822,6 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         return n > 0 ? max(0, min(n, kUnit * u1) - from) : 0;
     };
 
-    typedef float v4f __attribute__((ext_vector_type(4)));
     long long acc64 = 0;
     int fill = 0, nu = 0;   // list slots and units in use
     auto flush = [&]() {
@@ -821,7 +841,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
                 const float4 q0 = lists[wv][t++];
                 acc = fmaf(q0.w, de_term<kGauss>(p1, q0, ht, c2m1, gk), acc);
             }
-            acc64 += (long long)(acc * kFix);
+            acc64 += fix_round(acc);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_s_setprio(1);
@@ -845,7 +865,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     // charges packed at the start of its 64 slots; unit uo + j's slots are
     // [start, end) with (start << 16) | end in the unit table
     auto copy = [&](int k, int take, int o, int uo) {
-        const v4f* src = reinterpret_cast<const v4f*>(D.src) + k;
+        const __attribute__((address_space(1))) v4f* src = gmem(reinterpret_cast<const v4f*>(D.src)) + k;
         v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]) + o;
         for (int j = 0; j * kUnit < take; ++j) {
             const int u0 = kUnit * j, u1 = min(take, u0 + kUnit);
@@ -885,93 +905,14 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         }
     };
 
-    constexpr int kPer = 8;   // candidates per lane kept in registers
-    if (ncand <= 64 * kPer) {   // the same for the four waves (one group)
-        int* tab_n = &uends[0][0];   // 512 candidates' list lengths and starts (the first
-        int* tab_k = tab_n + 64 * kPer;   // 1,024 ints of the unit tables, before the unit ends take them)
-        static_assert(2 * 64 * kPer <= 4 * kList, "candidate table in the unit tables");
-        for (int ci = threadIdx.x; ci < 64 * kPer; ci += kSimBlock) {
-            int k0;
-            tab_n[ci] = brick(ci, k0);
-            tab_k[ci] = k0;
-        }
-        __syncthreads();
-        int n8[kPer], k8[kPer];
-        int tl = 0;
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            n8[j] = tab_n[64 * j + lane];
-            k8[j] = tab_k[64 * j + lane];
-            tl += n8[j];
-        }
-        __syncthreads();   // every wave holds its candidates before unit ends overwrite the table
-        const int T = wave_total(wave_incl_sum(tl));
+    // chunk s of S owns the slots [lo, hi) (T < 2^24 slots, so s T fits an int)
+    auto bounds = [&](int T) {
         if (s == 0 && lane == 0) D.tgrp[g] = max(1, T);
-        lo = int(((long long)s * T + S - 1) / S);
-        hi = int(((long long)(s + 1) * T + S - 1) / S);
-        int fr[kPer], td[kPer];
-        int pos = 0, mine = 0, munits = 0;
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int incl = wave_incl_sum(n8[j]);
-            td[j] = owned(n8[j], pos + incl - n8[j], fr[j]);
-            pos += wave_total(incl);
-            mine += td[j];
-            munits += (td[j] + kUnit - 1) / kUnit;
-        }
-        const int mincl = wave_incl_sum(mine);
-        if (wave_total(mincl) <= kList) {   // the whole chunk in one gather
-            // the chunk's units in lane and candidate order: (first slot << 7) | length
-            int uo = wave_incl_sum(munits) - munits;
-#pragma unroll
-            for (int j = 0; j < kPer; ++j)
-                for (int k = 0; k < td[j]; k += kUnit)
-                    uends[wv][uo++] = ((k8[j] + fr[j] + k) << 7) | min(kUnit, td[j] - k);
-            nu = wave_total(wave_incl_sum(munits));
-            __builtin_amdgcn_wave_barrier();
-            // every lane copies one slot of a unit, kGather units at a time (their
-            // loads in flight together); the near charges of each unit are packed
-            // in order (ballot prefix)
-            const v4f* src = reinterpret_cast<const v4f*>(D.src);
-            v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]);
-            int o = 0;
-            const v4f far = {-1e4f, -1e4f, -1e4f, 0.f};
-            for (int u = 0; u < nu; u += kGather) {
-                v4f a[kGather];
-                bool in[kGather];
-#pragma unroll
-                for (int j = 0; j < kGather; ++j) {
-                    const int d = u + j < nu ? __builtin_amdgcn_readfirstlane(uends[wv][u + j]) : 0;
-                    in[j] = lane < (d & 127);
-                    a[j] = in[j] ? src[(d >> 7) + lane] : far;
-                }
-#pragma unroll
-                for (int j = 0; j < kGather; ++j) {
-                    if (u + j >= nu) break;
-                    const bool k = in[j] && near(a[j]);
-                    const uint64_t m = __ballot(k);
-                    if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = a[j];
-                    const int n = __popcll(m);
-                    uends[wv][u + j] = (o << 16) | (o + n);
-                    o += n;
-                }
-            }
-            fill = o;
-        } else {
-#pragma unroll
-            for (int j = 0; j < kPer; ++j) segment(k8[j] + fr[j], td[j]);
-        }
-    } else {
-        // large boxes: T first, then the candidates again 64 at a time
-        int tl = 0;
-        for (int cb = 0; cb < ncand; cb += 64) {
-            int k0;
-            tl += brick(cb + lane, k0);
-        }
-        const int T = wave_total(wave_incl_sum(tl));
-        if (s == 0 && lane == 0) D.tgrp[g] = max(1, T);
-        lo = int(((long long)s * T + S - 1) / S);
-        hi = int(((long long)(s + 1) * T + S - 1) / S);
+        lo = int(unsigned(s * T + S - 1) / unsigned(S));
+        hi = int(unsigned((s + 1) * T + S - 1) / unsigned(S));
+    };
+    // the chunk from the candidates themselves, 64 at a time, in list segments
+    auto by_bricks = [&]() {
         int pos = 0;
         for (int cb = 0; cb < ncand && pos < hi; cb += 64) {
             int k0, from;
@@ -981,6 +922,111 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             pos += wave_total(incl);
             segment(k0 + from, todo);
         }
+    };
+    if (ncand <= 2 * kSimBlock) {   // the same for the four waves (one group)
+        // The workgroup tests the candidates once, two per thread, into a table of
+        // list lengths, list starts and positions (slots of the candidates before),
+        // in wave 3's list, which the gathers only overwrite after a barrier.
+        int* tab_n = reinterpret_cast<int*>(&lists[3][0]);
+        int* tab_k = tab_n + 2 * kSimBlock;
+        int* tab_p = tab_k + 2 * kSimBlock;
+        int* wtot = tab_p + 2 * kSimBlock;   // the waves' totals
+        static_assert(6 * kSimBlock + 4 <= 4 * kList, "candidate table in one wave's list");
+        int ka, kb;
+        const int c0 = 2 * int(threadIdx.x);
+        const int na = brick(c0, ka), nb = brick(c0 + 1, kb);
+        const int incl = wave_incl_sum(na + nb);
+        if (lane == 63) wtot[wv] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wv; ++w) before += wtot[w];
+        const int T = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        const int pa = before + incl - (na + nb);
+        reinterpret_cast<int2*>(tab_n)[threadIdx.x] = make_int2(na, nb);
+        reinterpret_cast<int2*>(tab_k)[threadIdx.x] = make_int2(ka, kb);
+        reinterpret_cast<int2*>(tab_p)[threadIdx.x] = make_int2(pa, pa + na);
+        __syncthreads();
+        bounds(T);
+        // the first candidate whose list ends after lo (the ends p + n never
+        // decrease): first the block of 8, then the candidate within it
+        auto end_at = [&](int ci) { return tab_p[ci] + tab_n[ci]; };
+        const int blk = __popcll(__ballot(end_at(8 * lane + 7) <= lo));
+        int cf = 8 * blk;
+        if (blk < 64) cf += __popcll(__ballot(lane < 8 && end_at(8 * blk + (lane & 7)) <= lo));
+        // the chunk's units, (first slot << 7) | length, in candidate order; a
+        // candidate's units start at p + 64 m and the chunk owns those in [lo, hi)
+        int total = 0;
+        for (int cb = cf; cb < 2 * kSimBlock; cb += 64) {
+            const int ci = cb + lane;
+            int n = 0, p = T, k0 = 0;
+            if (ci < 2 * kSimBlock) {
+                n = tab_n[ci];
+                p = tab_p[ci];
+                k0 = tab_k[ci];
+            }
+            const int m0 = (max(0, lo - p) + kUnit - 1) / kUnit;
+            const int m1 = (max(0, min(hi, p + n) - p) + kUnit - 1) / kUnit;
+            const int cnt = max(0, m1 - m0);
+            const int slots = cnt > 0 ? min(n, kUnit * m1) - kUnit * m0 : 0;
+            const int ui = wave_incl_sum(cnt), si = wave_incl_sum(slots);
+            int o = nu + ui - cnt;
+            for (int m = m0; m < m1; ++m, ++o)
+                if (o < kList) uends[wv][o] = ((k0 + kUnit * m) << 7) | min(kUnit, n - kUnit * m);
+            nu += wave_total(ui);
+            total += wave_total(si);
+            if (__builtin_amdgcn_readlane(p + n, 63) >= hi) break;   // the next batch starts at or past hi
+        }
+        __syncthreads();   // every wave is done with the table
+        if (total <= kList) {   // the whole chunk in one gather
+            __builtin_amdgcn_wave_barrier();
+            // every lane copies one slot of a unit, kGather units at a time (their
+            // loads in flight together); the near charges of each unit are packed
+            // in order (ballot prefix)
+            const __attribute__((address_space(1))) v4f* src = gmem(reinterpret_cast<const v4f*>(D.src));
+            v4f* dst = reinterpret_cast<v4f*>(&lists[wv][0]);
+            int o = 0;
+            for (int u0 = 0; u0 < nu; u0 += 64) {
+                const int nb64 = min(64, nu - u0);
+                const int dsc = lane < nb64 ? uends[wv][u0 + lane] : 0;   // unit u0 + lane
+                for (int u = 0; u < nb64; u += kGather) {
+                    v4f a[kGather];
+                    bool in[kGather];
+                    // every lane loads (lanes past a unit's length, or past the
+                    // last unit with descriptor 0, re-read a slot of the unit or
+                    // slot 0) and keeps only its own slot
+#pragma unroll
+                    for (int j = 0; j < kGather; ++j) {
+                        const int d = __builtin_amdgcn_readlane(dsc, (u + j) & 63);
+                        const int len = d & 127;
+                        in[j] = u + j < nb64 && lane < len;
+                        a[j] = src[unsigned(d >> 7) + unsigned(max(0, min(lane, len - 1)))];
+                    }
+#pragma unroll
+                    for (int j = 0; j < kGather; ++j) {
+                        if (u + j >= nb64) break;
+                        const bool k = in[j] && near(a[j]);
+                        const uint64_t m = __ballot(k);
+                        if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = a[j];
+                        const int n = __popcll(m);
+                        uends[wv][u0 + u + j] = (o << 16) | (o + n);
+                        o += n;
+                    }
+                }
+            }
+            fill = o;
+        } else {
+            nu = 0;
+            by_bricks();
+        }
+    } else {
+        // large boxes: T first, then the candidates again 64 at a time
+        int tl = 0;
+        for (int cb = 0; cb < ncand; cb += 64) {
+            int k0;
+            tl += brick(cb + lane, k0);
+        }
+        bounds(wave_total(wave_incl_sum(tl)));
+        by_bricks();
     }
     flush();
     // the workgroup's four chunks belong to the same 64 targets: their totals are
