@@ -322,7 +322,8 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
         hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream,
                            P, none);
         if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
-        const dim3 grid((D.items_cap + 3) / 4 + 1);
+        // work items, the refill workgroup, then the decisions' inputs (decide_prep)
+        const dim3 grid((D.items_cap + 3) / 4 + 1 + D.nblk);
         if (D.gauss)
             hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P, none);
         else
@@ -833,7 +834,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
             (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
             (st = sim_alloc(c, D.torder, cap)) || (st = sim_alloc(c, D.tgrp, D.ngrp)) ||
             (st = sim_alloc(c, D.items, 4 * size_t(c->cull_quads_alloc))) || (st = sim_alloc(c, D.nitems, 1)) ||
-            (st = sim_alloc(c, D.isum, cap)))
+            (st = sim_alloc(c, D.isum, cap)) || (st = sim_alloc(c, D.drec, cap)))
             return st;
         HIPCHK(c, hipMemset(D.bmask, 0, size_t(nb) * 32 * 4));
         HIPCHK(c, hipMemset(D.scnt, 0, size_t(nb) * 4));
@@ -1345,6 +1346,12 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
     std::sort(order.begin(), order.end());
     for (const auto& o : order)
         for (int x = 0; x < cull[o.second].items_cap / 4; ++x) de.push_back(make_int2(o.second, x));
+    // and last each replica's decide_prep workgroups (short: they fill slots
+    // that the work items free)
+    for (int j = 0; j < b->n_cull; ++j) {
+        const SimDev& D = cull[j];
+        for (int x = 0; x < D.nblk; ++x) de.push_back(make_int2(j, D.items_cap / 4 + 1 + x));
+    }
     const int need = int(std::max(emit.size(), de.size()));
     if (need > b->de_map_cap) {
         if (b->d_emit_map) (void)hipFree(b->d_emit_map);

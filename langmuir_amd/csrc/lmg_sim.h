@@ -51,6 +51,18 @@ struct SimCtl {
     long long step;
 };
 
+// The decision inputs of one carrier that do not depend on its dE, made by
+// cull_de_kernel's last workgroups while the pair sums run (decide_prep)
+struct DecideRec {
+    double u;          // the decision draw's uniform (draw set)
+    double pd0;        // q (V_bg(fs) - V_bg(site))
+    double cpl;        // coupling of the hop
+    double bf, bs;     // binding terms at fs and at site
+    int site, fs, life, path;
+    int flags;         // bit 0: draws, bits 8..15: own-grid type at fs
+    int pad;
+};
+
 struct SimDev {
     Dims d;
     int hop, sim_type, coulomb, src_metro, src_coul, balance, rec_range, gauss;
@@ -85,6 +97,7 @@ struct SimDev {
     int* nitems;
     int items_cap;
     long long* isum;              // per target: fixed-point pair sum (int64 atomics)
+    DecideRec* drec;              // per carrier: decide_kernel's inputs but dE
     unsigned long long lookahead;
     int* e_site[2];
     int* e_life[2];
@@ -713,6 +726,76 @@ __device__ __forceinline__ int fdiv(int t, int d, float inv, int& r) {
     return q;
 }
 
+// decideFuture's inputs but dE (chargeagent.cpp:90-161) for carrier block blk
+// (256 carriers), in the same order and arithmetic as decide_kernel's own
+// path: the block's draws start at rng_pos plus the draws of the blocks before
+// it, and a drawing carrier takes the next engine output.  Reads only state
+// that the step's dE phase leaves alone, so a repeated dE phase rewrites the
+// same records.  wsum: 4 ints and a long long of LDS.
+__device__ void decide_prep(const SimDev& D, int blk, int* wsum, long long* pre) {
+    const SimCtl& C = *D.ctl;
+    const int ne = C.n_e, N = ne + C.n_h, cur = C.cur;
+    if (blk * kSimBlock >= N) return;   // whole workgroup
+    const int i = blk * kSimBlock + int(threadIdx.x);
+    const bool valid = i < N;
+    const bool isE = i < ne;
+    const int li = isE ? i : i - ne;
+    const int q = isE ? -1 : 1;
+    const long long draws_lane = wg_prefix_loads(D.blk_draw, blk, 1, 0);
+    DecideRec r{};
+    int draw = 0;
+    uint8_t tf = kEmpty;
+    double vb_fs = 0.0, vb_site = 0.0;
+    if (valid) {
+        r.site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
+        r.fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+        r.life = isE ? D.e_life[cur][li] : D.h_life[cur][li];
+        r.path = isE ? D.e_path[cur][li] : D.h_path[cur][li];
+        draw = D.flag[i];
+        tf = isE ? D.etype[r.fs] : D.htype[r.fs];
+        vb_fs = D.vbg[r.fs];
+        vb_site = D.vbg[r.site];
+        // bindingPotential (chargeagent.cpp:463-481)
+        if (isE) {
+            r.bf = D.htype[r.fs] == kHole ? D.self_int + D.binding : 0.0;
+            r.bs = D.htype[r.site] == kHole ? D.self_int + D.binding : 0.0;
+        } else {
+            r.bf = D.etype[r.fs] == kElectron ? D.self_int - D.binding : 0.0;
+            r.bs = D.etype[r.site] == kElectron ? D.self_int - D.binding : 0.0;
+        }
+    }
+    const long long before = wg_prefix_total(draws_lane, pre);
+    int total;
+    const int off = wg_scan<kSimBlock>(draw, wsum, total);
+    if (!valid) return;
+    if (draw) {
+        r.u = u01(rng_word(D, C.rng_pos + (unsigned long long)(before + off)));
+        if (tf == kEmpty) {
+            double pd = vb_fs - vb_site;
+            pd *= q;
+            r.pd0 = pd;
+            int x1, y1, z1, x2, y2, z2;
+            decode(D.d, r.site, x1, y1, z1);
+            decode(D.d, r.fs, x2, y2, z2);
+            const int h1 = D.hop + 1;
+            r.cpl = D.coupling[(abs(x1 - x2) * h1 + abs(y1 - y2)) * h1 + abs(z1 - z2)];
+        }
+    }
+    r.flags = draw | (int(tf) << 8);
+    D.drec[i] = r;
+}
+
+// one record as four 16-byte global loads
+__device__ __forceinline__ DecideRec load_rec(const DecideRec* p) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(DecideRec) == 4 * sizeof(v4i), "record of four 16-byte words");
+    const __attribute__((address_space(1))) v4i* w = gmem(reinterpret_cast<const v4i*>(p));
+    const v4i a[4] = {w[0], w[1], w[2], w[3]};
+    DecideRec r;
+    __builtin_memcpy(&r, a, sizeof(r));
+    return r;
+}
+
 constexpr int kList = 512;   // LDS charge slots per wave
 constexpr int kGather = 8;   // units whose loads a wave keeps in flight together
 constexpr int kMaxCullS = 32;   // chunks per target group
@@ -750,6 +833,11 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     __shared__ int uends[4][kList];   // (start << 16) | end of each unit in the list
     if (wx == (D.items_cap + 3) / 4) {
         mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
+        return;
+    }
+    if (wx > (D.items_cap + 3) / 4) {   // the last nblk workgroups: the decisions' inputs
+        decide_prep(D, wx - (D.items_cap + 3) / 4 - 1, &uends[0][0],
+                    reinterpret_cast<long long*>(&uends[1][0]));
         return;
     }
     // Every wave starts with a chain of dependent loads (its box, candidate
@@ -1070,6 +1158,72 @@ __global__ __launch_bounds__(kSimBlock) void decide_kernel(const SimDev* __restr
     const bool isE = i < ne;
     const int li = isE ? i : i - ne;
     const int q = isE ? -1 : 1;
+    if (D.cull && !only_de) {
+        // the culled dE phase made every input but dE (decide_prep): the record
+        // and the pair sum are the only loads, and no barrier comes before the
+        // decisions
+        if (threadIdx.x < 4) { att[threadIdx.x] = 0; suc[threadIdx.x] = 0; }
+        __syncthreads();
+        int code = 0, keep = 1;
+        if (valid) {
+            const DecideRec r = load_rec(D.drec + i);
+            const long long is = *gmem(&D.isum[i]);
+            {   // the step is done with this carrier's bitmask word and brick counters
+                int x, y, z;
+                decode(D.d, r.site, x, y, z);
+                uint32_t bit;
+                D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)] = 0u;
+                D.scnt[cull_brick(D, x, y, z)] = 0;
+            }
+            const double sde = double(is) * kFixInv;
+            const double de = q * (D.pref * sde - D.self_int * q + r.bf - r.bs);
+            D.de[i] = de;
+            int fs = r.fs;
+            if (r.flags & 1) {
+                const double u = r.u;
+                if ((r.flags >> 8) == kEmpty) {
+                    const double pd = r.pd0 + de;
+                    // metropolisWithCoupling (rand.cpp:91-106)
+                    const bool acc = (pd > 0.0) ? (r.cpl * exp(-pd * D.beta) > u) : (r.cpl > u);
+                    code = acc ? 1 : 0;
+                } else {   // drain: DrainAgent::tryToAccept -> chooseYes(rate) (drainagent.cpp:83-92)
+                    const int* ds = isE ? D.e_drain : D.h_drain;
+                    const int k = (fs == ds[0]) ? 0 : 1;
+                    const int flux = isE ? (LMG_FLUX_EDL + k) : (LMG_FLUX_HDL + k);
+                    const int slot = (isE ? 0 : 2) + k;
+                    atomicAdd(&att[slot], 1u);
+                    if (D.rate[flux] > u) {
+                        atomicAdd(&suc[slot], 1u);
+                        code = 2;
+                    }
+                }
+            }
+            (isE ? D.e_life[cur] : D.h_life[cur])[li] = r.life + 1;
+            if (code) (isE ? D.e_path[cur] : D.h_path[cur])[li] = r.path + 1;
+            else fs = r.site;
+            if (isE) D.e_fsite[li] = fs; else D.h_fsite[li] = fs;
+            D.code[i] = code;
+            D.removed[i] = 0;
+            keep = (code != 2);
+            if (code == 1) {
+                const unsigned long long key = ((unsigned long long)(uint32_t)(C.step + 1) << 32) |
+                                               (unsigned long long)(0xffffffffu - uint32_t(li));
+                atomicMax(isE ? &D.claim_e[fs] : &D.claim_h[fs], key);
+            }
+        }
+        const int ke = __syncthreads_count(valid && isE && keep);
+        const int kh = __syncthreads_count(valid && !isE && keep);
+        if (threadIdx.x == 0) {
+            D.blk_keep[2 * blockIdx.x] = ke;
+            D.blk_keep[2 * blockIdx.x + 1] = kh;
+        }
+        if (threadIdx.x < 4) {
+            const int flux = threadIdx.x < 2 ? LMG_FLUX_EDL + threadIdx.x : LMG_FLUX_HDL + (threadIdx.x - 2);
+            if (att[threadIdx.x]) atomicAdd(&D.ctl->att[flux], (unsigned long long)att[threadIdx.x]);
+            if (suc[threadIdx.x]) atomicAdd(&D.ctl->suc[flux], (unsigned long long)suc[threadIdx.x]);
+        }
+        return;
+    }
     // draws of the earlier workgroups: loads issued here, summed after the
     // carrier's own loads
     const long long draws_lane = only_de ? 0 : wg_prefix_loads(D.blk_draw, blockIdx.x, 1, 0);
