@@ -310,6 +310,20 @@ unsigned long long next_pow2(unsigned long long v) {
     return p;
 }
 
+// The culled dE kernel over a descriptor array: the table form (dynamic LDS:
+// the largest table) when every culled descriptor has a table, else f is
+// evaluated in the kernel (1/r, or the erf-screened form).
+void launch_cull_de(dim3 grid, hipStream_t st, const SimDev* P, const int2* map, bool table, bool gauss,
+                    int max_rtab_n) {
+    const dim3 blk(kSimBlock);
+    if (table)
+        hipLaunchKernelGGL((cull_de_kernel<false, true>), grid, blk, cull_lds_bytes(max_rtab_n), st, P, map);
+    else if (gauss)
+        hipLaunchKernelGGL((cull_de_kernel<true, false>), grid, blk, cull_lds_bytes(0), st, P, map);
+    else
+        hipLaunchKernelGGL((cull_de_kernel<false, false>), grid, blk, cull_lds_bytes(0), st, P, map);
+}
+
 // dE pair sums + decision flags + random-stream refill (always launched: with
 // Coulomb off only the flags and the refill run).  `mid`, when given, is
 // recorded before the pair-sum kernel itself (after the culled path's list
@@ -319,15 +333,12 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
     const SimDev* P = c->d_sim;
     if (D.cull) {
         const int2* none = nullptr;   // one replica: no workgroup map
-        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream,
+        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 2), dim3(kSimBlock), 0, c->stream,
                            P, none);
         if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
         // work items, the refill workgroup, then the decisions' inputs (decide_prep)
         const dim3 grid((D.items_cap + 3) / 4 + 1 + D.nblk);
-        if (D.gauss)
-            hipLaunchKernelGGL((cull_de_kernel<true>), grid, dim3(kSimBlock), 0, c->stream, P, none);
-        else
-            hipLaunchKernelGGL((cull_de_kernel<false>), grid, dim3(kSimBlock), 0, c->stream, P, none);
+        launch_cull_de(grid, c->stream, P, none, D.rtab_n > 0, D.gauss != 0, D.rtab_n);
         return launch_check(c, "cull_de_kernel");
     }
     if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
@@ -527,15 +538,18 @@ lmg_status lmg_create(const lmg_params* params, lmg_ctx** out) {
     c->dev = p.device_id;
     hipError_t e = hipSetDevice(c->dev);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    auto drop = [c]() {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    };
     if (e != hipSuccess) {
         g_create_error = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
-        delete c;
+        drop();
         return LMG_ERR_HIP;
     }
     if (scratch_setup(c) != LMG_OK) {
         g_create_error = c->err;
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+        drop();
         return LMG_ERR_HIP;
     }
     *out = c;
@@ -791,8 +805,15 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         D.ngrp = (cap + kCullGroup - 1) / kCullGroup;
         int ncu = 0, per_cu = 0;
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+        // the pair-function table (LDS of cull_de_kernel) for cutoffs up to 63
+        const long long c2 = (long long)c->p.cutoff * c->p.cutoff;
+        D.rtab_n = c2 + 1 <= kTabMax ? int(c2 + 1) : 0;
+        D.rtab_lim = D.rtab_n > 0 ? uint32_t(4 * c2) : 0u;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                      &per_cu, reinterpret_cast<const void*>(&cull_de_kernel<false>), kSimBlock, 0));
+                      &per_cu,
+                      D.rtab_n > 0 ? reinterpret_cast<const void*>(&cull_de_kernel<false, true>)
+                                   : reinterpret_cast<const void*>(&cull_de_kernel<false, false>),
+                      kSimBlock, cull_lds_bytes(D.rtab_n)));
         // work items (workgroups of four chunks of one target group): to start
         // with, as many as keep every workgroup of cull_de_kernel resident at
         // once (one wave of workgroups, no tail), at least one per group;
@@ -927,6 +948,19 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         D.bperm = bperm;
         D.dstart = dstart;
         D.dsrc = dsrc;
+        if (D.rtab_n > 0) {
+            // f(r) = 1/r or erf(r / (sqrt2 sigma)) / r in FP64, rounded once to FP32
+            // (potential.cpp:351-382, 430-470); 0 at r = 0 and at r^2 = c^2
+            std::vector<float> tv(size_t(D.rtab_n), 0.f);
+            for (int q = 1; q + 1 < D.rtab_n; ++q) {
+                const double rr = std::sqrt(double(q));
+                tv[q] = float(c->p.gaussian_sigma > 0 ? std::erf(gk_d * rr) / rr : 1.0 / rr);
+            }
+            float* rt = nullptr;
+            if ((st = sim_alloc(c, rt, tv.size()))) return st;
+            HIPCHK(c, hipMemcpy(rt, tv.data(), tv.size() * 4, hipMemcpyHostToDevice));
+            D.rtab = rt;
+        }
     }
     D.ctl = c->ctl;
     HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
@@ -970,6 +1004,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
     C.rng_pos = k;
     C.rng_step0 = k;
     C.gen_pos = k + 624ull;
+    C.ring_base = k;
     for (int i = 0; i < 10; ++i) { C.att[i] = s->flux_attempts[i]; C.suc[i] = s->flux_successes[i]; }
     C.step = s->current_step;
     HIPCHK(c, hipMemcpy(c->ctl, &C, sizeof(C), hipMemcpyHostToDevice));
@@ -1191,6 +1226,8 @@ struct lmg_batch {
     double tuned_us = -1.0;   // the dE phase of every replica at the chosen shapes
     std::string report;       // every timed candidate (JSON)
     bool coulomb = false, gauss = false, any_recomb = false;
+    bool cull_table = false;  // the culled replicas' dE kernel in table form
+    int cull_tab_n = 0;       // ... and its table entries (the largest replica's)
     bool use_graphs = true;
     hipGraphExec_t step_exec = nullptr;   // one step of every replica, captured at the first step
     std::vector<hipEvent_t> events;
@@ -1228,12 +1265,8 @@ void batch_launch_de(lmg_batch* b) {
         }
         hipLaunchKernelGGL(cull_emit_kernel, dim3(b->emit_wgs), blk, 0, st, b->d_cull,
                            (const int2*)b->d_emit_map);
-        if (b->gauss)
-            hipLaunchKernelGGL((cull_de_kernel<true>), dim3(b->de_wgs), blk, 0, st, b->d_cull,
-                               (const int2*)b->d_de_map);
-        else
-            hipLaunchKernelGGL((cull_de_kernel<false>), dim3(b->de_wgs), blk, 0, st, b->d_cull,
-                               (const int2*)b->d_de_map);
+        launch_cull_de(dim3(b->de_wgs), st, b->d_cull, (const int2*)b->d_de_map, b->cull_table, b->gauss,
+                       b->cull_tab_n);
     }
     if (b->n_pairs) {
         const dim3 g_de(b->max_nblk_p + 1, b->coulomb ? b->max_S_p : 1, b->n_pairs);
@@ -1323,15 +1356,21 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
     }
     b->n_pairs = int(pairs.size());
     b->n_cull = int(cull.size());
+    b->cull_table = b->n_cull > 0;
+    b->cull_tab_n = 0;
+    for (const SimDev& D : cull) {
+        b->cull_table = b->cull_table && D.rtab_n > 0;
+        b->cull_tab_n = std::max(b->cull_tab_n, D.rtab_n);
+    }
     BHIPCHK(b, hipMemcpy(b->d_sims, all.data(), sizeof(SimDev) * n, hipMemcpyHostToDevice));
     if (b->n_pairs) BHIPCHK(b, hipMemcpy(b->d_pairs, pairs.data(), sizeof(SimDev) * b->n_pairs, hipMemcpyHostToDevice));
     if (!b->n_cull) return LMG_OK;
     BHIPCHK(b, hipMemcpy(b->d_cull, cull.data(), sizeof(SimDev) * b->n_cull, hipMemcpyHostToDevice));
-    // cull_emit_kernel: replica by replica, ceil(nbrick / 4) + nblk + 1 workgroups each
+    // cull_emit_kernel: replica by replica, ceil(nbrick / 4) + nblk + 2 workgroups each
     std::vector<int2> emit, de;
     for (int j = 0; j < b->n_cull; ++j) {
         const SimDev& D = cull[j];
-        for (int x = 0, e = (D.nbrick + 3) / 4 + D.nblk + 1; x < e; ++x) emit.push_back(make_int2(j, x));
+        for (int x = 0, e = (D.nbrick + 3) / 4 + D.nblk + 2; x < e; ++x) emit.push_back(make_int2(j, x));
     }
     // cull_de_kernel: every replica's refill workgroup first (they run the
     // longest serial loop), then the work items, the replicas with the most
@@ -1393,6 +1432,14 @@ lmg_status batch_plan(lmg_batch* b, int de_path) {
         b->any_recomb |= (D.sim_type == 1 && D.rate[LMG_FLUX_XD] > 0);
         max_charges = std::max(max_charges, D.cap_e + D.cap_h + D.n_d_eff);
     }
+    // one form of the culled kernel serves every culled replica: when some of the
+    // cullable replicas have a pair-function table, only those are culled, so
+    // each culled replica sums as its own context does
+    bool any_tab = false;
+    for (int r = 0; r < n; ++r) any_tab = any_tab || (capable[r] && b->reps[r]->sim.rtab_n > 0);
+    if (any_tab)
+        for (int r = 0; r < n; ++r)
+            if (b->reps[r]->sim.rtab_n == 0) capable[r] = 0;
     BHIPCHK(b, hipMalloc(&b->d_sims, sizeof(SimDev) * n));
     BHIPCHK(b, hipMalloc(&b->d_pairs, sizeof(SimDev) * n));
     BHIPCHK(b, hipMalloc(&b->d_cull, sizeof(SimDev) * n));

@@ -45,6 +45,7 @@ struct SimCtl {
     unsigned long long rng_pos;   // engine outputs consumed before the pending decisions
     unsigned long long rng_step0; // ... before the pending step's proposals
     unsigned long long gen_pos;   // raw words x[0 .. gen_pos) generated
+    unsigned long long ring_base; // the first word the ring holds (the loaded window's)
     unsigned long long draws_dec; // decision draws of the pending step
     unsigned long long draws_rec; // recombination draws of the pending step
     unsigned long long att[10], suc[10];
@@ -72,6 +73,12 @@ struct SimDev {
     int nblk;                     // workgroups of kSimBlock carriers at capacity
     uint32_t c2m1;
     float gk;
+    // culled dE: the pair function f by r^2 (1/r, or erf(r / (sqrt2 sigma)) / r),
+    // FP64 values rounded to FP32, for r^2 = 0 .. c^2 with f = 0 at r^2 = 0 and
+    // at c^2; null when c^2 + 1 > kTabMax (the kernel then evaluates f itself)
+    const float* rtab;
+    uint32_t rtab_lim;            // 4 c^2: byte offset of the last (zero) entry
+    int rtab_n;                   // c^2 + 1 entries
     double vl, vr, beta, pref, self_int, binding;
     double rate[10];
     double coupling[27];
@@ -218,7 +225,8 @@ __device__ __forceinline__ long long wg_prefix_total(long long s, long long* slo
 
 // ---------------------------------------------------------------------------
 // MT19937 refill: x[m] = x[m-227] ^ twist(x[m-624], x[m-623]), so 227
-// consecutive words can be produced at once from the last 624.
+// consecutive words can be produced at once from the last 624 (623 with the
+// recurrence expanded, below).
 // ---------------------------------------------------------------------------
 // Boost's uniform_int(0, r) rejects a word w iff w >= (r + 1) * bucket(r), which is at
 // least 2^32 - 1 - r.  A neighbour list holds at most 34 entries (hopping.range 2 in
@@ -226,29 +234,110 @@ __device__ __forceinline__ long long wg_prefix_total(long long s, long long* slo
 // rejected; mt_refill lists the engine outputs at or above it.
 constexpr uint32_t kRejectZone = 0xffffffffu - 33u;
 constexpr int kDangerSlots = 16;
+constexpr int kEmitRefillNs = 4000;   // wall time of cull_emit_kernel's refill workgroup
 
-__device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [1024] */, unsigned long long target) {
-    unsigned long long gen = D.ctl->gen_pos;
-    if (gen >= target) return;
-    for (int t = threadIdx.x; t < 624; t += blockDim.x) {
-        const unsigned long long m = gen - 624ull + t;
-        tail[m & 1023ull] = D.ring[m & D.ring_mask];
+// The recurrence x[m] = x[m-227] ^ tw(x[m-624], x[m-623]) expanded twice,
+//   x[m] = x[m-681] ^ tw(x[m-1078], x[m-1077]) ^ tw(x[m-851], x[m-850]) ^ tw(x[m-624], x[m-623]),
+// reaches back at least 623 words, so a round makes 623 words at once (454 with
+// one expansion, 227 with none, while the ring holds less history).
+//
+// The last 2048 words sit in LDS twice over: word m at slots m mod 2048 and
+// m mod 2048 + 2048.  Every read of a round, m - k for 227 <= k <= 1078, is then
+// slot (m mod 2048) + 2048 - k, one base address per word and fixed offsets
+// (no wrap, no per-read address arithmetic).  Within a round the slots written
+// and the slots read are never the same: the words are less than 2048 apart.
+__device__ __forceinline__ uint32_t mt_tw(uint32_t x0, uint32_t x1) {
+    const uint32_t y = (x0 & 0x80000000u) | (x1 & 0x7fffffffu);
+    return (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+constexpr uint32_t kTailMask = 2047u;
+constexpr int kTailWords = 4096;   // LDS words of the refill's window (two copies)
+// A workgroup barrier that orders LDS only: __syncthreads() also waits for
+// every outstanding global store (its release fence covers global memory),
+// which would cost each round a store round trip for the ring words.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// One round of kF + 1 expansions (kF = 0: 227 words, 1: 454, 2: 623) by a
+// workgroup of kThreads: each thread issues the LDS reads of all its words
+// before any arithmetic, so that a round costs one LDS latency for its reads
+// and one for its writes.  g = gen mod 2^32 (the ring holds fewer than 2^32
+// words, so ring and tail indices follow from the low 32 bits).
+typedef __attribute__((address_space(1))) uint32_t gu32;
+template <int kThreads, int kF>
+__device__ __forceinline__ void mt_round(const SimDev& D, uint32_t* tail, uint32_t g, gu32* ring, uint32_t rmask) {
+    constexpr int kW = kF == 2 ? 623 : (kF == 1 ? 454 : 227);
+    constexpr int kPer = (kW + kThreads - 1) / kThreads;
+    constexpr int kR = 3 + 2 * kF;   // LDS reads per word
+    constexpr int kOff[7] = {624, 623, kF == 2 ? 681 : (kF == 1 ? 454 : 227), 851, 850, 1078, 1077};
+    uint32_t a[kPer][7];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = int(threadIdx.x) + kThreads * k;
+        if (t < kW) {
+            const uint32_t* w = tail + ((g + uint32_t(t)) & kTailMask) + 2048;
+#pragma unroll
+            for (int r = 0; r < kR; ++r) a[k][r] = w[-kOff[r]];
+        }
     }
-    __syncthreads();
-    while (gen < target) {
-        if (threadIdx.x < 227) {
-            const unsigned long long m = gen + threadIdx.x;
-            const uint32_t v = mt_next(tail[(m - 624ull) & 1023ull], tail[(m - 623ull) & 1023ull],
-                                       tail[(m - 227ull) & 1023ull]);
-            tail[m & 1023ull] = v;
-            D.ring[m & D.ring_mask] = v;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = int(threadIdx.x) + kThreads * k;
+        if (t < kW) {
+            const uint32_t m = g + uint32_t(t);
+            uint32_t v = mt_tw(a[k][0], a[k][1]) ^ a[k][2];
+            if (kF >= 1) v ^= mt_tw(a[k][3], a[k][4]);
+            if (kF == 2) v ^= mt_tw(a[k][5], a[k][6]);
+            tail[m & kTailMask] = v;
+            tail[(m & kTailMask) + 2048] = v;
+            ring[m & rmask] = v;   // a global store: a flat one would hold up every LDS wait
             if (mt_temper(v) >= kRejectZone) {   // engine output m - 624 (about one in 10^8)
                 const int slot = atomicAdd(&D.ctl->danger_n, 1);
-                D.ctl->danger[slot % kDangerSlots] = m - 624ull;
+                // (the full index: gen_pos - (gen_pos - m) mod 2^32)
+                const unsigned long long gp = D.ctl->gen_pos;
+                D.ctl->danger[slot % kDangerSlots] = gp + (unsigned long long)(uint32_t(m - uint32_t(gp))) - 624ull;
             }
         }
-        gen += 227ull;
-        __syncthreads();
+    }
+}
+// Refill up to `target` with rounds of at most kMaxF + 1 expansions: a
+// workgroup that shares its CU with LDS-bound pair sums runs the 227-word
+// rounds (three LDS reads a word) faster than the wide ones (seven).  A
+// nonzero deadline (s_memrealtime, 100 MHz) stops after the first round that
+// ends past it: thread 0 decides, the workgroup reads the decision after the
+// round's barrier (two flags, alternating, so that the next round's write
+// cannot overtake a slow reader).
+template <int kThreads, int kMaxF>
+__device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [kTailWords] */, unsigned long long target,
+                          unsigned long long deadline = 0, int* stop = nullptr /* LDS [2] */) {
+    unsigned long long gen = D.ctl->gen_pos;
+    if (gen >= target) return;
+    const unsigned long long base = D.ctl->ring_base;
+    gu32* ring = (gu32*)D.ring;   // (address-space cast)
+    const uint32_t rmask = uint32_t(D.ring_mask);
+    for (unsigned long long m = gen - min(gen - base, 1078ull) + threadIdx.x; m < gen; m += kThreads) {
+        const uint32_t v = ring[uint32_t(m) & rmask];
+        tail[m & kTailMask] = v;
+        tail[(m & kTailMask) + 2048] = v;
+    }
+    lds_barrier();
+    for (int round = 0; gen < target; ++round) {
+        const unsigned long long hist = gen - base;
+        if (kMaxF >= 2 && hist >= 1078ull) {
+            mt_round<kThreads, 2>(D, tail, uint32_t(gen), ring, rmask);
+            gen += 623ull;
+        } else if (kMaxF >= 1 && hist >= 851ull) {
+            mt_round<kThreads, 1>(D, tail, uint32_t(gen), ring, rmask);
+            gen += 454ull;
+        } else {
+            mt_round<kThreads, 0>(D, tail, uint32_t(gen), ring, rmask);
+            gen += 227ull;
+        }
+        if (deadline && threadIdx.x == 0) stop[round & 1] = __builtin_amdgcn_s_memrealtime() >= deadline;
+        lds_barrier();
+        if (deadline && stop[round & 1]) break;
     }
     if (threadIdx.x == 0) D.ctl->gen_pos = gen;
 }
@@ -314,7 +403,8 @@ __device__ __forceinline__ float4 charge_f4(const SimDev& D, int j, int nd, int 
 // smaller replicas exit surplus workgroups).  Workgroup (x, s) sums charge
 // chunk s for carriers [256x, 256x + 256); chunk-0 workgroups also flag the
 // carriers whose decision draws (own-grid type Empty or Drain at the proposed
-// site, chargeagent.cpp:95-157).  Workgroup (nblk, 0) refills the random stream.
+// site, chargeagent.cpp:95-157).  Workgroup (nblk, 0) refills the random
+// stream.
 //
 // For each charge j it accumulates q_j ( [0<r2<c2] f(r2) - [0<r1<c2] f(r1) ),
 // the pair part of p2 - p1 in ChargeAgent::coulombCPU (chargeagent.cpp:215-271).
@@ -379,10 +469,10 @@ template <bool kGauss, bool kPairs>
 __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];     // replica
     __shared__ float4 tile[256];
-    __shared__ uint32_t tail[1024];
+    __shared__ uint32_t tail[kTailWords];
     if (int(blockIdx.x) >= D.nblk || int(blockIdx.y) >= D.S) {
         if (int(blockIdx.x) == D.nblk && blockIdx.y == 0)
-            mt_refill(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
+            mt_refill<kSimBlock, 2>(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     const SimCtl& C = *D.ctl;
@@ -644,7 +734,8 @@ __device__ __forceinline__ int2 cull_workgroup(const int2* __restrict__ map) {
     return map ? map[blockIdx.x] : make_int2(0, int(blockIdx.x));
 }
 
-// per replica ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
+// per replica ceil(nbrick / 4) + nblk + 2 workgroups (the last two: the work
+// items, and the first part of the random-stream refill)
 __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp,
                                                               const int2* __restrict__ map) {
     const int2 wg = cull_workgroup(map);
@@ -656,6 +747,16 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
     if (wx == nbw + D.nblk) {   // cull_de_kernel's work items
         __shared__ int wsum[4];
         cull_items(D, wsum);
+        return;
+    }
+    if (wx == nbw + D.nblk + 1) {
+        // The random-stream refill, for at most kEmitRefillNs: the grid of this
+        // kernel leaves most CUs free, so the rounds run without the LDS
+        // contention of the pair sums; cull_de_kernel completes what is left.
+        __shared__ uint32_t tail[kTailWords];
+        __shared__ int stop[2];
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        mt_refill<kSimBlock, 2>(D, tail, D.ctl->rng_pos + 624ull + D.lookahead, t0 + kEmitRefillNs / 10, stop);
         return;
     }
     if (wx >= nbw) {   // targets: proposals, decision-draw flags, their brick's slots
@@ -796,14 +897,78 @@ __device__ __forceinline__ DecideRec load_rec(const DecideRec* p) {
     return r;
 }
 
-constexpr int kList = 512;   // LDS charge slots per wave
+// ---------------------------------------------------------------------------
+// Table form of the culled dE terms.  Every squared distance on the lattice
+// is an integer, so f(r) is a table lookup by r^2: the in-cutoff test, the
+// r = 0 exclusion and (for the erf-screened pair function) erf all fold into
+// the table, whose entries are 0 at r^2 = 0 and at c^2, and indices beyond
+// c^2 are clamped to c^2.  The workgroup copies the table into LDS.
+//
+// 4 r^2 comes out exact from integer dot products.  Coordinates are taken
+// relative to an origin o (the centre of the group's box); a charge q is
+// staged as the record
+//   w0 = (-8 qx, -8 qy), w1 = (-8 qz, 4 |q|^2) as int16 pairs, w2 = 4 |q|^2,
+//   w3 = its charge (FP32 bits)
+// and a target p holds (px, py), (pz, 1), (pz, 0) as int16 pairs and 4 |p|^2:
+//   4 |p - q|^2 = 4|p|^2 + 4|q|^2 - 8 p.q
+//              = dot2((px, py), w0, dot2((pz, 1), w1, 4|p|^2))      if 4|q|^2 < 2^15,
+//              = dot2((px, py), w0, dot2((pz, 0), w1, w2 + 4|p|^2)) always
+// (v_dot2_i32_i16: two int16 products plus an int32).  Both forms give the same
+// integer; the short one needs every kept charge within 90 sites of o.  A term
+// is then f[4r2^2] - f[4r1^2] (two LDS reads, byte offsets), times q.
+// The table entries are FP64 values rounded once, so a term's error is at most
+// that of two rounded values; over a 64-term unit the FP32 accumulation
+// dominates, as in the arithmetic form.
+// ---------------------------------------------------------------------------
+constexpr int kTabMax = 4096;   // table entries (16 KB of LDS): cutoffs up to 63
+typedef short v2s16 __attribute__((ext_vector_type(2)));
+struct TabTarget {
+    v2s16 xy, z1, z0;
+    int n4;
+};
+__device__ __forceinline__ v2s16 pk16(int lo, int hi) {
+    return __builtin_bit_cast(v2s16, (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16));
+}
+// n4 = 4 |p|^2 plus the table's LDS byte address, so that the dot products give
+// the entry's address directly
+__device__ __forceinline__ TabTarget tab_target(const float4& p, int ox, int oy, int oz, uint32_t base) {
+    const int x = int(p.x) - ox, y = int(p.y) - oy, z = int(p.z) - oz;
+    return TabTarget{pk16(x, y), pk16(z, 1), pk16(z, 0), int(base) + 4 * (x * x + y * y + z * z)};
+}
+// v_dot2_i32_i16 in its three-operand (VOP3P) form, d = a.lo b.lo + a.hi b.hi + c.
+// (__builtin_amdgcn_sdot2 is selected as the accumulate-in-place form, which
+// costs a copy of a loop-invariant accumulator per use.)  An ALU instruction
+// only: no memory access.
+__device__ __forceinline__ int sdot2(v2s16 a, int b, int c) {
+    int d;
+    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+template <bool kShort>
+__device__ __forceinline__ int tab_r2x4(const TabTarget& p, int w0, int w1, int w2) {
+    const int zq = kShort ? sdot2(p.z1, w1, p.n4) : sdot2(p.z0, w1, w2 + p.n4);
+    return sdot2(p.xy, w0, zq);
+}
+// f at LDS byte address a = base + 4 r^2 (beyond the cutoff, the last entry at lim = base + 4 c^2)
+__device__ __forceinline__ float tab_f(int a, uint32_t lim) {
+    typedef const __attribute__((address_space(3))) float* lds_cfp;
+    return *reinterpret_cast<lds_cfp>(uintptr_t(min(uint32_t(a), lim)));
+}
+
+constexpr int kList = 448;   // LDS charge slots per wave
 constexpr int kGather = 8;   // units whose loads a wave keeps in flight together
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
+// cull_de_kernel's dynamic LDS: a table of rtab_n entries (0: no table), then
+// four wave lists of kList float4 slots and their kList-entry unit tables
+inline size_t cull_lds_bytes(int rtab_n) {
+    return 16 * size_t((rtab_n + 3) / 4) + 4 * size_t(kList) * (sizeof(float4) + sizeof(int));
+}
 
-// Per replica ceil(items_cap / 4) + 1 workgroups: workgroup x takes work item x
-// (group g, quad q of Q) and its wave w chunk s = 4 q + w of S = 4 Q; the last
-// workgroup refills the random stream.
+// Per replica ceil(items_cap / 4) + 1 + nblk workgroups: workgroup x takes
+// work item x (group g, quad q of Q) and its wave w chunk s = 4 q + w of
+// S = 4 Q; the next one completes the random-stream refill; the last nblk
+// prepare the decisions' inputs (decide_prep).
 //
 // Every wave of group g finds the same work: the bricks of the box widened by
 // the cutoff whose distance to the box is below the cutoff, their lists cut
@@ -823,21 +988,30 @@ constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's lis
 // and the units a group skips hold only terms that are exactly 0 for its
 // targets; integer addition is exact, so each target's total is the same
 // whatever its group, chunk count or order.
-template <bool kGauss>
+template <bool kGauss, bool kTable>
 __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp,
                                                             const int2* __restrict__ map) {
     const int2 wg = cull_workgroup(map);
     const SimDev& D = Dp[wg.x];
     const int wx = wg.y;
-    __shared__ float4 lists[4][kList];
-    __shared__ int uends[4][kList];   // (start << 16) | end of each unit in the list
+    // LDS (dynamic, cull_lds_bytes): [pair-function table | 4 wave lists | their
+    // unit tables]; the table at address 0, so that a lookup's address is the
+    // clamped index itself
+    extern __shared__ float4 cull_lds[];
+    const int tab16 = kTable ? (D.rtab_n + 3) >> 2 : 0;
+    float4 (*lists)[kList] = reinterpret_cast<float4 (*)[kList]>(cull_lds + tab16);
+    int (*uends)[kList] = reinterpret_cast<int (*)[kList]>(cull_lds + tab16 + 4 * kList);
     if (wx == (D.items_cap + 3) / 4) {
-        mt_refill(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
+        // Completes the refill cull_emit_kernel began.  Its CU is LDS bound
+        // (the pair sums), where the 227-word rounds (three LDS reads a word)
+        // run faster than the wide ones; its serial rounds issue first.
+        static_assert(3 * kList * sizeof(float4) >= kTailWords * sizeof(uint32_t), "refill tail in lists 0-2");
+        __builtin_amdgcn_s_setprio(3);
+        mt_refill<kSimBlock, 0>(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     if (wx > (D.items_cap + 3) / 4) {   // the last nblk workgroups: the decisions' inputs
-        decide_prep(D, wx - (D.items_cap + 3) / 4 - 1, &uends[0][0],
-                    reinterpret_cast<long long*>(&uends[1][0]));
+        decide_prep(D, wx - (D.items_cap + 3) / 4 - 1, &uends[0][0], reinterpret_cast<long long*>(&uends[1][0]));
         return;
     }
     // Every wave starts with a chain of dependent loads (its box, candidate
@@ -859,6 +1033,19 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     // a group holds at least one target: the lanes past the last load its sites
     // (no branch between the loads of the chain)
     const int i = gmem(D.torder)[min(r, nt - 1)];
+    // the pair-function table: the loads of its first 4 kSimBlock entries go out
+    // with the chain's, and the values are written to LDS once the target sites
+    // are in (read after the first workgroup barrier)
+    float* rtab_lds = reinterpret_cast<float*>(cull_lds);
+    const __attribute__((address_space(1))) float* rt = gmem(D.rtab);
+    float tv[4];
+    if (kTable) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = int(threadIdx.x) + kSimBlock * j;
+            tv[j] = k < D.rtab_n ? rt[k] : 0.f;
+        }
+    }
     const v4f ta = gmem(reinterpret_cast<const v4f*>(D.tgt1))[i];
     const v4f tb = gmem(reinterpret_cast<const v4f*>(D.tgt2))[i];
     const float4 p1 = make_float4(ta.x, ta.y, ta.z, ta.w), p2 = make_float4(tb.x, tb.y, tb.z, tb.w);
@@ -877,11 +1064,67 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     const int by0 = max(0, y0 - c + 1) >> 3, by1 = min(D.d.Y - 1, y1 + c - 1) >> 3;
     const int bz0 = max(0, z0 - c + 1) >> 3, bz1 = min(D.d.Z - 1, z1 + c - 1) >> 3;
     const int nx = bx1 - bx0 + 1, ny = by1 - by0 + 1, ncand = nx * ny * (bz1 - bz0 + 1);
+    if (kTable) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = int(threadIdx.x) + kSimBlock * j;
+            if (k < D.rtab_n) rtab_lds[k] = tv[j];
+        }
+        for (int k = int(threadIdx.x) + 4 * kSimBlock; k < D.rtab_n; k += kSimBlock) rtab_lds[k] = rt[k];
+    }
     // estimates for fdiv, which corrects a quotient that is off by one
     const float inx = __builtin_amdgcn_rcpf(float(nx)), iny = __builtin_amdgcn_rcpf(float(ny));
     const HopTerm ht = hop_term(p1, p2);
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
+    // table form: coordinates relative to the box's centre o; a kept charge lies
+    // within c + |H| of o (H: the box's half extents), so the short form of
+    // 4 r^2 holds when |H| <= 90 - c
+    const int ox = (x0 + x1) >> 1, oy = (y0 + y1) >> 1, oz = (z0 + z1) >> 1;
+    typedef __attribute__((address_space(3))) float* lds_fp;
+    const uint32_t tbase = uint32_t(uintptr_t((lds_fp)rtab_lds));   // (address-space cast)
+    const TabTarget ta1 = tab_target(p1, ox, oy, oz, tbase), ta2 = tab_target(p2, ox, oy, oz, tbase);
+    const int hx = max(x1 - ox, ox - x0), hy = max(y1 - oy, oy - y0), hz = max(z1 - oz, oz - z0);
+    const bool short_r2 = c <= 90 && hx * hx + hy * hy + hz * hz <= (90 - c) * (90 - c);
+    const uint32_t tlim = tbase + D.rtab_lim;
+    // a charge as the wave's list holds it (the record above in table form)
+    auto stage = [&](const v4f& a) -> v4f {
+        if (!kTable) return a;
+        const int qx = int(a.x) - ox, qy = int(a.y) - oy, qz = int(a.z) - oz;
+        const int n4 = 4 * (qx * qx + qy * qy + qz * qz);
+        v4i w;
+        w.x = int((uint32_t(-8 * qx) & 0xffffu) | (uint32_t(-8 * qy) << 16));
+        w.y = int((uint32_t(-8 * qz) & 0xffffu) | (uint32_t(n4) << 16));
+        w.z = n4;
+        w.w = __float_as_int(a.w);
+        return __builtin_bit_cast(v4f, w);
+    };
+    // the FP32 sum of list slots [t, e) in table form: q (f[4 r2^2] - f[4 r1^2]),
+    // four slots' loads at a time, added in slot order
+    auto tab_sum = [&](auto kshort, int t, int e) {
+        constexpr bool kS = decltype(kshort)::value;
+        const v4i* L = reinterpret_cast<const v4i*>(&lists[wv][0]);
+        float acc = 0.f;
+        for (; t + 4 <= e; t += 4) {
+            v4i w[4];
+            float d[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = L[t + j];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                d[j] = tab_f(tab_r2x4<kS>(ta2, w[j].x, w[j].y, w[j].z), tlim) -
+                       tab_f(tab_r2x4<kS>(ta1, w[j].x, w[j].y, w[j].z), tlim);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc = fmaf(__int_as_float(w[j].w), d[j], acc);
+        }
+        for (; t < e; ++t) {
+            const v4i w = L[t];
+            const float d = tab_f(tab_r2x4<kS>(ta2, w.x, w.y, w.z), tlim) -
+                            tab_f(tab_r2x4<kS>(ta1, w.x, w.y, w.z), tlim);
+            acc = fmaf(__int_as_float(w.w), d, acc);
+        }
+        return acc;
+    };
     const __attribute__((address_space(1))) int* sstart = gmem(D.sstart);
     // candidate ci -> its list [k0, k0 + n) when kept, else n = 0
     auto brick = [&](int ci, int& k0) -> int {
@@ -915,7 +1158,15 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     auto flush = [&]() {
         __builtin_amdgcn_s_setprio(0);
         __builtin_amdgcn_wave_barrier();
-        for (int q = 0; q < nu; ++q) {
+        if (kTable) {
+            for (int q = 0; q < nu; ++q) {
+                const int se = __builtin_amdgcn_readfirstlane(uends[wv][q]);
+                const float acc = short_r2 ? tab_sum(std::true_type{}, se >> 16, se & 0xffff)
+                                           : tab_sum(std::false_type{}, se >> 16, se & 0xffff);
+                acc64 += fix_round(acc);
+            }
+        }
+        for (int q = 0; q < (kTable ? 0 : nu); ++q) {
             const int se = __builtin_amdgcn_readfirstlane(uends[wv][q]);
             int t = se >> 16;
             const int e = se & 0xffff;
@@ -964,11 +1215,11 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
                 for (int i = 0; i < 8; ++i) a[i] = src[u + i];
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
-                    if (near(a[i])) dst[w++] = a[i];
+                    if (near(a[i])) dst[w++] = stage(a[i]);
             }
             for (; u < u1; ++u) {
                 const v4f a = src[u];
-                if (near(a)) dst[w++] = a;
+                if (near(a)) dst[w++] = stage(a);
             }
             uends[wv][uo + j] = ((o + u0) << 16) | (o + w);
         }
@@ -1094,7 +1345,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
                         if (u + j >= nb64) break;
                         const bool k = in[j] && near(a[j]);
                         const uint64_t m = __ballot(k);
-                        if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = a[j];
+                        if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = stage(a[j]);
                         const int n = __popcll(m);
                         uends[wv][u0 + u + j] = (o << 16) | (o + n);
                         o += n;
@@ -1107,7 +1358,9 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             by_bricks();
         }
     } else {
-        // large boxes: T first, then the candidates again 64 at a time
+        // large boxes: T first, then the candidates again 64 at a time (the four
+        // waves share the box, so they all come here; the barrier publishes the table)
+        if (kTable) __syncthreads();
         int tl = 0;
         for (int cb = 0; cb < ncand; cb += 64) {
             int k0;
@@ -1384,7 +1637,7 @@ struct OneShared {
     int bal_go, bal_src;
     unsigned int att[10], suc[10];
     double red[16];
-    uint32_t tail[1024];
+    uint32_t tail[kTailWords];
     // carriers injected this step on the culled path: their bricks
     int inj_n;
     int inj_b[kInjMax];
@@ -1794,7 +2047,7 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
         }
     }
     __syncthreads();
-    if (mode == 1) mt_refill(D, sh.tail, sh.pos + 624ull + D.lookahead);
+    if (mode == 1) mt_refill<kOneBlock, 2>(D, sh.tail, sh.pos + 624ull + D.lookahead);
     else inject_all(D, sh);
     __syncthreads();
     const int ne = sh.n_e, N = ne + sh.n_h, cur = sh.cur;

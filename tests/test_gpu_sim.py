@@ -155,6 +155,29 @@ def test_culled_brick_orders(gpu, oracle, name, over):
     assert sim.device_error() == 0
 
 
+@pytest.mark.parametrize("name,cutoff", [
+    # the largest cutoff whose pair-function table fits (c^2 + 1 = 3970 entries)
+    ("transistor2d", 63), ("gauss3d", 63),
+    # one more: the culled kernel evaluates 1/r (and erf) itself
+    ("transistor2d", 64), ("gauss3d", 64),
+])
+def test_culled_table_limit(gpu, oracle, name, cutoff):
+    """The culled dE kernel looks the pair function up in an LDS table indexed by
+    r^2 for cutoffs up to 63 and evaluates it in the kernel beyond; both forms,
+    Coulomb and erf-screened, give the oracle's dE and trajectory."""
+    w, sim, sp = make(oracle, gpu, name, "culled", electrostatic_cutoff=cutoff)
+    fe, fh, de, dh = sim.delta_e()
+    w.propose()
+    w.coulomb()
+    e, h = w.carriers(0), w.carriers(1)
+    assert np.array_equal(fe, e["fsite"]) and np.array_equal(fh, h["fsite"])
+    for got, ref in ((de, e["de"]), (dh, h["de"])):
+        assert np.all(np.abs(got - ref) <= de_tolerance(ref))
+    w, sim, sp = make(oracle, gpu, name, "culled", electrostatic_cutoff=cutoff)
+    assert run_parity(w, sim, 40, sp.inverse_kt) <= 2
+    assert sim.device_error() == 0
+
+
 def test_solar_full_size_delta_e(gpu, oracle):
     """128x128x32 with 5242 electrons + 5242 holes, cutoff 30: proposals bit-exact,
     dE within tolerance for all 10,484 carriers."""
