@@ -333,7 +333,7 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
     const SimDev* P = c->d_sim;
     if (D.cull) {
         const int2* none = nullptr;   // one replica: no workgroup map
-        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 2), dim3(kSimBlock), 0, c->stream,
+        hipLaunchKernelGGL(cull_emit_kernel, dim3((D.nbrick + 3) / 4 + D.nblk + 1), dim3(kSimBlock), 0, c->stream,
                            P, none);
         if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
         // work items, the refill workgroup, then the decisions' inputs (decide_prep)
@@ -1366,11 +1366,11 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
     if (b->n_pairs) BHIPCHK(b, hipMemcpy(b->d_pairs, pairs.data(), sizeof(SimDev) * b->n_pairs, hipMemcpyHostToDevice));
     if (!b->n_cull) return LMG_OK;
     BHIPCHK(b, hipMemcpy(b->d_cull, cull.data(), sizeof(SimDev) * b->n_cull, hipMemcpyHostToDevice));
-    // cull_emit_kernel: replica by replica, ceil(nbrick / 4) + nblk + 2 workgroups each
+    // cull_emit_kernel: replica by replica, ceil(nbrick / 4) + nblk + 1 workgroups each
     std::vector<int2> emit, de;
     for (int j = 0; j < b->n_cull; ++j) {
         const SimDev& D = cull[j];
-        for (int x = 0, e = (D.nbrick + 3) / 4 + D.nblk + 2; x < e; ++x) emit.push_back(make_int2(j, x));
+        for (int x = 0, e = (D.nbrick + 3) / 4 + D.nblk + 1; x < e; ++x) emit.push_back(make_int2(j, x));
     }
     // cull_de_kernel: every replica's refill workgroup first (they run the
     // longest serial loop), then the work items, the replicas with the most

@@ -252,5 +252,11 @@ template <class T>
 __device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
     return (const __attribute__((address_space(1))) T*)p;
 }
+// The same for stores and atomics: a flat store holds up every later LDS wait
+// (a workgroup barrier included) until it completes; a global one does not.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gst(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
 
 }  // namespace lmg

@@ -210,7 +210,7 @@ __device__ void wg_scan2(int a, int b, int* wsum, int& ea, int& eb, int& ta, int
 __device__ __forceinline__ long long wg_prefix_loads(const int* arr, int nb, int stride, int off) {
     long long s = 0;
     if (threadIdx.x < 64)
-        for (int b = threadIdx.x; b < nb; b += 64) s += arr[stride * b + off];
+        for (int b = threadIdx.x; b < nb; b += 64) s += gmem(arr)[stride * b + off];
     return s;
 }
 // (the counts are carriers: their sums fit an int)
@@ -234,7 +234,6 @@ __device__ __forceinline__ long long wg_prefix_total(long long s, long long* slo
 // rejected; mt_refill lists the engine outputs at or above it.
 constexpr uint32_t kRejectZone = 0xffffffffu - 33u;
 constexpr int kDangerSlots = 16;
-constexpr int kEmitRefillNs = 4000;   // wall time of cull_emit_kernel's refill workgroup
 
 // The recurrence x[m] = x[m-227] ^ tw(x[m-624], x[m-623]) expanded twice,
 //   x[m] = x[m-681] ^ tw(x[m-1078], x[m-1077]) ^ tw(x[m-851], x[m-850]) ^ tw(x[m-624], x[m-623]),
@@ -272,15 +271,15 @@ __device__ __forceinline__ void mt_round(const SimDev& D, uint32_t* tail, uint32
     constexpr int kPer = (kW + kThreads - 1) / kThreads;
     constexpr int kR = 3 + 2 * kF;   // LDS reads per word
     constexpr int kOff[7] = {624, 623, kF == 2 ? 681 : (kF == 1 ? 454 : 227), 851, 850, 1078, 1077};
+    // (unconditional: the slots of a thread past the round's words are inside the
+    // window too, and their values go unused; one basic block, so that every
+    // read is issued before the first wait)
     uint32_t a[kPer][7];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-        const int t = int(threadIdx.x) + kThreads * k;
-        if (t < kW) {
-            const uint32_t* w = tail + ((g + uint32_t(t)) & kTailMask) + 2048;
+        const uint32_t* w = tail + ((g + threadIdx.x + uint32_t(kThreads * k)) & kTailMask) + 2048;
 #pragma unroll
-            for (int r = 0; r < kR; ++r) a[k][r] = w[-kOff[r]];
-        }
+        for (int r = 0; r < kR; ++r) a[k][r] = w[-kOff[r]];
     }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -304,14 +303,9 @@ __device__ __forceinline__ void mt_round(const SimDev& D, uint32_t* tail, uint32
 }
 // Refill up to `target` with rounds of at most kMaxF + 1 expansions: a
 // workgroup that shares its CU with LDS-bound pair sums runs the 227-word
-// rounds (three LDS reads a word) faster than the wide ones (seven).  A
-// nonzero deadline (s_memrealtime, 100 MHz) stops after the first round that
-// ends past it: thread 0 decides, the workgroup reads the decision after the
-// round's barrier (two flags, alternating, so that the next round's write
-// cannot overtake a slow reader).
+// rounds (three LDS reads a word) faster than the wide ones (seven).
 template <int kThreads, int kMaxF>
-__device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [kTailWords] */, unsigned long long target,
-                          unsigned long long deadline = 0, int* stop = nullptr /* LDS [2] */) {
+__device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [kTailWords] */, unsigned long long target) {
     unsigned long long gen = D.ctl->gen_pos;
     if (gen >= target) return;
     const unsigned long long base = D.ctl->ring_base;
@@ -323,7 +317,7 @@ __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [kTailWords] */
         tail[(m & kTailMask) + 2048] = v;
     }
     lds_barrier();
-    for (int round = 0; gen < target; ++round) {
+    while (gen < target) {
         const unsigned long long hist = gen - base;
         if (kMaxF >= 2 && hist >= 1078ull) {
             mt_round<kThreads, 2>(D, tail, uint32_t(gen), ring, rmask);
@@ -335,9 +329,7 @@ __device__ void mt_refill(const SimDev& D, uint32_t* tail /* LDS [kTailWords] */
             mt_round<kThreads, 0>(D, tail, uint32_t(gen), ring, rmask);
             gen += 227ull;
         }
-        if (deadline && threadIdx.x == 0) stop[round & 1] = __builtin_amdgcn_s_memrealtime() >= deadline;
         lds_barrier();
-        if (deadline && stop[round & 1]) break;
     }
     if (threadIdx.x == 0) D.ctl->gen_pos = gen;
 }
@@ -581,9 +573,10 @@ __device__ __forceinline__ void cull_mark(const SimDev& D, bool isE, int li, int
     int x, y, z;
     decode(D.d, site, x, y, z);
     uint32_t bit;
-    atomicOr(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)], bit);
-    atomicAdd(&D.scnt[cull_brick(D, x, y, z)], 1);
-    (isE ? D.e_who : D.h_who)[site] = li;
+    __hip_atomic_fetch_or(gst(&D.bmask[cull_word(D, x, y, z, isE ? 0 : 1, bit)]), bit, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(gst(&D.scnt[cull_brick(D, x, y, z)]), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *gst(&(isE ? D.e_who : D.h_who)[site]) = li;
 }
 
 // Offsets of the charge lists (linear brick order; sst[0 .. nb]) and of the
@@ -734,8 +727,7 @@ __device__ __forceinline__ int2 cull_workgroup(const int2* __restrict__ map) {
     return map ? map[blockIdx.x] : make_int2(0, int(blockIdx.x));
 }
 
-// per replica ceil(nbrick / 4) + nblk + 2 workgroups (the last two: the work
-// items, and the first part of the random-stream refill)
+// per replica ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
 __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp,
                                                               const int2* __restrict__ map) {
     const int2 wg = cull_workgroup(map);
@@ -747,16 +739,6 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
     if (wx == nbw + D.nblk) {   // cull_de_kernel's work items
         __shared__ int wsum[4];
         cull_items(D, wsum);
-        return;
-    }
-    if (wx == nbw + D.nblk + 1) {
-        // The random-stream refill, for at most kEmitRefillNs: the grid of this
-        // kernel leaves most CUs free, so the rounds run without the LDS
-        // contention of the pair sums; cull_de_kernel completes what is left.
-        __shared__ uint32_t tail[kTailWords];
-        __shared__ int stop[2];
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        mt_refill<kSimBlock, 2>(D, tail, D.ctl->rng_pos + 624ull + D.lookahead, t0 + kEmitRefillNs / 10, stop);
         return;
     }
     if (wx >= nbw) {   // targets: proposals, decision-draw flags, their brick's slots
@@ -1002,9 +984,9 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     float4 (*lists)[kList] = reinterpret_cast<float4 (*)[kList]>(cull_lds + tab16);
     int (*uends)[kList] = reinterpret_cast<int (*)[kList]>(cull_lds + tab16 + 4 * kList);
     if (wx == (D.items_cap + 3) / 4) {
-        // Completes the refill cull_emit_kernel began.  Its CU is LDS bound
-        // (the pair sums), where the 227-word rounds (three LDS reads a word)
-        // run faster than the wide ones; its serial rounds issue first.
+        // The random-stream refill.  Its CU is LDS bound (the pair sums), where
+        // the 227-word rounds (three LDS reads a word) run faster than the wide
+        // ones; its serial rounds issue first.
         static_assert(3 * kList * sizeof(float4) >= kTailWords * sizeof(uint32_t), "refill tail in lists 0-2");
         __builtin_amdgcn_s_setprio(3);
         mt_refill<kSimBlock, 0>(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
@@ -1727,20 +1709,20 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
     const SimDev& D = Dp[blockIdx.z];   // replica
     __shared__ int wsum[8];
     __shared__ long long pre_e, pre_h;
-    const SimCtl& C = *D.ctl;
+    const __attribute__((address_space(1))) SimCtl& C = *gmem(D.ctl);
     const int ne = C.n_e, nh = C.n_h, N = ne + nh, cur = C.cur, nxt = cur ^ 1;
     const int nb = (N + kSimBlock - 1) / kSimBlock;
     if (blockIdx.x == 0 && threadIdx.x < 64) {
         int se = 0, sh_ = 0;
         for (int b = threadIdx.x; b < nb; b += 64) {
-            se += D.blk_keep[2 * b];
-            sh_ += D.blk_keep[2 * b + 1];
+            se += gmem(D.blk_keep)[2 * b];
+            sh_ += gmem(D.blk_keep)[2 * b + 1];
         }
         se = wave_total(wave_incl_sum(se));
         sh_ = wave_total(wave_incl_sum(sh_));
         if (threadIdx.x == 0) {
-            D.ctl->n_e_next = int(se);
-            D.ctl->n_h_next = int(sh_);
+            gst(D.ctl)->n_e_next = int(se);
+            gst(D.ctl)->n_h_next = int(sh_);
         }
     }
     if (int(blockIdx.x) * kSimBlock >= N) return;
@@ -1754,20 +1736,20 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
     const long long keep_h_lane = wg_prefix_loads(D.blk_keep, blockIdx.x, 2, 1);
     int keep_e = 0, keep_h = 0, dest = 0, life = 0, path = 0;
     if (valid) {
-        const int site = isE ? D.e_site[cur][li] : D.h_site[cur][li];
-        const int code = D.code[i];
-        const bool rem = D.removed[i] != 0;
-        life = isE ? D.e_life[cur][li] : D.h_life[cur][li];
-        path = isE ? D.e_path[cur][li] : D.h_path[cur][li];
-        uint8_t* ty = isE ? D.etype : D.htype;
+        const int site = *gmem(isE ? &D.e_site[cur][li] : &D.h_site[cur][li]);
+        const int code = gmem(D.code)[i];
+        const bool rem = gmem(D.removed)[i] != 0;
+        life = *gmem(isE ? &D.e_life[cur][li] : &D.h_life[cur][li]);
+        path = *gmem(isE ? &D.e_path[cur][li] : &D.h_path[cur][li]);
+        __attribute__((address_space(1))) uint8_t* ty = gst(isE ? D.etype : D.htype);
         dest = site;
         if (rem || code == 2) {
             ty[site] = kEmpty;
         } else if (code == 1) {
-            const int fs = isE ? D.e_fsite[li] : D.h_fsite[li];
+            const int fs = *gmem(isE ? &D.e_fsite[li] : &D.h_fsite[li]);
             const unsigned long long key = ((unsigned long long)(uint32_t)(C.step + 1) << 32) |
                                            (unsigned long long)(0xffffffffu - uint32_t(li));
-            if ((isE ? D.claim_e[fs] : D.claim_h[fs]) == key) {
+            if (*gmem(isE ? &D.claim_e[fs] : &D.claim_h[fs]) == key) {
                 ty[site] = kEmpty;
                 ty[fs] = isE ? kElectron : kHole;
                 dest = fs;
@@ -1783,15 +1765,15 @@ __global__ __launch_bounds__(kSimBlock) void tick_kernel(const SimDev* __restric
     wg_scan2<kSimBlock>(keep_e, keep_h, wsum, oe, oh, te, th);
     if (keep_e) {
         const int w = int(base_e) + oe;
-        D.e_site[nxt][w] = dest;
-        D.e_life[nxt][w] = life;
-        D.e_path[nxt][w] = path;
+        gst(D.e_site[nxt])[w] = dest;
+        gst(D.e_life[nxt])[w] = life;
+        gst(D.e_path[nxt])[w] = path;
         if (D.cull) cull_mark(D, true, w, dest);   // the next step's brick lists
     } else if (keep_h) {
         const int w = int(base_h) + oh;
-        D.h_site[nxt][w] = dest;
-        D.h_life[nxt][w] = life;
-        D.h_path[nxt][w] = path;
+        gst(D.h_site[nxt])[w] = dest;
+        gst(D.h_life[nxt])[w] = life;
+        gst(D.h_path[nxt])[w] = path;
         if (D.cull) cull_mark(D, false, w, dest);
     }
 }
