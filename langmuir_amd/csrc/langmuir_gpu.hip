@@ -310,6 +310,20 @@ unsigned long long next_pow2(unsigned long long v) {
     return p;
 }
 
+// The all-pairs dE kernel over a descriptor array (grid z: its length): the
+// table form when every descriptor has a table, else f evaluated in the kernel
+void launch_de(dim3 grid, hipStream_t st, const SimDev* P, bool coulomb, bool table, bool gauss) {
+    const dim3 blk(kSimBlock);
+    if (!coulomb)
+        hipLaunchKernelGGL((de_kernel<false, false, false>), grid, blk, kDeLdsBytes, st, P);
+    else if (table)
+        hipLaunchKernelGGL((de_kernel<false, true, true>), grid, blk, kDeLdsBytes, st, P);
+    else if (gauss)
+        hipLaunchKernelGGL((de_kernel<true, true, false>), grid, blk, kDeLdsBytes, st, P);
+    else
+        hipLaunchKernelGGL((de_kernel<false, true, false>), grid, blk, kDeLdsBytes, st, P);
+}
+
 // The culled dE kernel over a descriptor array: the table form (dynamic LDS:
 // the largest table) when every culled descriptor has a table, else f is
 // evaluated in the kernel (1/r, or the erf-screened form).
@@ -342,12 +356,7 @@ lmg_status sim_launch_de(lmg_ctx* c, hipEvent_t mid = nullptr) {
         return launch_check(c, "cull_de_kernel");
     }
     if (mid) HIPCHK(c, hipEventRecord(mid, c->stream));
-    if (!D.coulomb)
-        hipLaunchKernelGGL((de_kernel<false, false>), dim3(D.nblk + 1, 1), dim3(kSimBlock), 0, c->stream, P);
-    else if (D.gauss)
-        hipLaunchKernelGGL((de_kernel<true, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
-    else
-        hipLaunchKernelGGL((de_kernel<false, true>), dim3(D.nblk + 1, D.S), dim3(kSimBlock), 0, c->stream, P);
+    launch_de(dim3(D.nblk + 1, D.coulomb ? D.S : 1), c->stream, P, D.coulomb != 0, D.rtab_n > 0, D.gauss != 0);
     return launch_check(c, "de_kernel");
 }
 
@@ -800,15 +809,17 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         const bool worth = frac(d.X) * frac(d.Y) * frac(d.Z) < 0.6 && double(cap) * (cap + D.n_d_eff) >= 8e6;
         D.cull = c->de_mode == 2 || (c->de_mode == 0 && worth);
     }
+    // the pair-function table (LDS of the dE kernels) for cutoffs up to 63
+    {
+        const long long c2 = (long long)c->p.cutoff * c->p.cutoff;
+        D.rtab_n = D.coulomb && c2 + 1 <= kTabMax ? int(c2 + 1) : 0;
+        D.rtab_lim = D.rtab_n > 0 ? uint32_t(4 * c2) : 0u;
+    }
     if (D.cull) {
         D.cut = c->p.cutoff;
         D.ngrp = (cap + kCullGroup - 1) / kCullGroup;
         int ncu = 0, per_cu = 0;
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-        // the pair-function table (LDS of cull_de_kernel) for cutoffs up to 63
-        const long long c2 = (long long)c->p.cutoff * c->p.cutoff;
-        D.rtab_n = c2 + 1 <= kTabMax ? int(c2 + 1) : 0;
-        D.rtab_lim = D.rtab_n > 0 ? uint32_t(4 * c2) : 0u;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                       &per_cu,
                       D.rtab_n > 0 ? reinterpret_cast<const void*>(&cull_de_kernel<false, true>)
@@ -948,19 +959,19 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         D.bperm = bperm;
         D.dstart = dstart;
         D.dsrc = dsrc;
-        if (D.rtab_n > 0) {
-            // f(r) = 1/r or erf(r / (sqrt2 sigma)) / r in FP64, rounded once to FP32
-            // (potential.cpp:351-382, 430-470); 0 at r = 0 and at r^2 = c^2
-            std::vector<float> tv(size_t(D.rtab_n), 0.f);
-            for (int q = 1; q + 1 < D.rtab_n; ++q) {
-                const double rr = std::sqrt(double(q));
-                tv[q] = float(c->p.gaussian_sigma > 0 ? std::erf(gk_d * rr) / rr : 1.0 / rr);
-            }
-            float* rt = nullptr;
-            if ((st = sim_alloc(c, rt, tv.size()))) return st;
-            HIPCHK(c, hipMemcpy(rt, tv.data(), tv.size() * 4, hipMemcpyHostToDevice));
-            D.rtab = rt;
+    }
+    if (D.rtab_n > 0) {
+        // f(r) = 1/r or erf(r / (sqrt2 sigma)) / r in FP64, rounded once to FP32
+        // (potential.cpp:351-382, 430-470); 0 at r = 0 and at r^2 = c^2
+        std::vector<float> tv(size_t(D.rtab_n), 0.f);
+        for (int q = 1; q + 1 < D.rtab_n; ++q) {
+            const double rr = std::sqrt(double(q));
+            tv[q] = float(c->p.gaussian_sigma > 0 ? std::erf(gk_d * rr) / rr : 1.0 / rr);
         }
+        float* rt = nullptr;
+        if ((st = sim_alloc(c, rt, tv.size()))) return st;
+        HIPCHK(c, hipMemcpy(rt, tv.data(), tv.size() * 4, hipMemcpyHostToDevice));
+        D.rtab = rt;
     }
     D.ctl = c->ctl;
     HIPCHK(c, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
@@ -1212,8 +1223,8 @@ struct lmg_batch {
     SimDev* d_sims = nullptr;   // every replica (the commit kernels)
     int max_nblk = 0;
     // all-pairs replicas: their descriptors, de_kernel's grid
-    SimDev* d_pairs = nullptr;
-    int n_pairs = 0, max_nblk_p = 0, max_S_p = 0;
+    SimDev* d_pairs = nullptr;   // those with a pair-function table first
+    int n_pairs = 0, n_pairs_tab = 0, max_nblk_p = 0, max_S_p = 0;
     // culled replicas: their descriptors and the workgroup maps of the two
     // culled kernels (launch workgroup -> (culled replica, its workgroup))
     SimDev* d_cull = nullptr;
@@ -1269,10 +1280,11 @@ void batch_launch_de(lmg_batch* b) {
                        b->cull_tab_n);
     }
     if (b->n_pairs) {
-        const dim3 g_de(b->max_nblk_p + 1, b->coulomb ? b->max_S_p : 1, b->n_pairs);
-        if (!b->coulomb) hipLaunchKernelGGL((de_kernel<false, false>), g_de, blk, 0, b->stream, b->d_pairs);
-        else if (b->gauss) hipLaunchKernelGGL((de_kernel<true, true>), g_de, blk, 0, b->stream, b->d_pairs);
-        else hipLaunchKernelGGL((de_kernel<false, true>), g_de, blk, 0, b->stream, b->d_pairs);
+        // the replicas with a pair-function table first, then the others
+        const int nt = std::min(b->n_pairs_tab, b->n_pairs), na = b->n_pairs - nt;
+        const int gy = b->coulomb ? b->max_S_p : 1;
+        if (nt) launch_de(dim3(b->max_nblk_p + 1, gy, nt), b->stream, b->d_pairs, b->coulomb, true, b->gauss);
+        if (na) launch_de(dim3(b->max_nblk_p + 1, gy, na), b->stream, b->d_pairs + nt, b->coulomb, false, b->gauss);
         if (b->n_cull) {   // join the side stream
             (void)hipEventRecord(b->join, b->side);
             (void)hipStreamWaitEvent(b->stream, b->join, 0);
@@ -1332,7 +1344,7 @@ bool cull_candidate(const lmg_ctx* c) {
 // take the same path.
 lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, const std::vector<int>& quads) {
     const int n = int(b->reps.size());
-    std::vector<SimDev> all(n), pairs, cull;
+    std::vector<SimDev> all(n), pairs, pairs_ar, cull;
     std::vector<int> cull_rep;
     b->max_nblk_p = 0;
     b->max_S_p = 1;
@@ -1346,7 +1358,7 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
             cull_rep.push_back(r);
         } else {
             D.S = std::max(1, std::min(D.S_pairs, S));
-            pairs.push_back(D);
+            (D.coulomb && D.rtab_n == 0 ? pairs_ar : pairs).push_back(D);
             b->max_nblk_p = std::max(b->max_nblk_p, D.nblk);
             b->max_S_p = std::max(b->max_S_p, D.S);
         }
@@ -1354,6 +1366,8 @@ lmg_status batch_apply(lmg_batch* b, const std::vector<char>& culled, int S, con
         b->paths[r] = D.cull ? 2 : 1;
         BHIPCHK(b, hipMemcpy(c->d_sim, &D, sizeof(SimDev), hipMemcpyHostToDevice));
     }
+    b->n_pairs_tab = int(pairs.size());
+    pairs.insert(pairs.end(), pairs_ar.begin(), pairs_ar.end());
     b->n_pairs = int(pairs.size());
     b->n_cull = int(cull.size());
     b->cull_table = b->n_cull > 0;

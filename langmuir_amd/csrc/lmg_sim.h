@@ -457,14 +457,79 @@ __device__ __forceinline__ long long fix_round(float acc) {
     return (long long)(b - 0x4338000000000000ull);
 }
 
-template <bool kGauss, bool kPairs>
+// ---------------------------------------------------------------------------
+// Table form of the dE terms (both dE kernels).  Every squared distance on the lattice
+// is an integer, so f(r) is a table lookup by r^2: the in-cutoff test, the
+// r = 0 exclusion and (for the erf-screened pair function) erf all fold into
+// the table, whose entries are 0 at r^2 = 0 and at c^2, and indices beyond
+// c^2 are clamped to c^2.  The workgroup copies the table into LDS.
+//
+// 4 r^2 comes out exact from integer dot products.  Coordinates are taken
+// relative to an origin o (the centre of the group's box); a charge q is
+// staged as the record
+//   w0 = (-8 qx, -8 qy), w1 = (-8 qz, 4 |q|^2) as int16 pairs, w2 = 4 |q|^2,
+//   w3 = its charge (FP32 bits)
+// and a target p holds (px, py), (pz, 1), (pz, 0) as int16 pairs and 4 |p|^2:
+//   4 |p - q|^2 = 4|p|^2 + 4|q|^2 - 8 p.q
+//              = dot2((px, py), w0, dot2((pz, 1), w1, 4|p|^2))      if 4|q|^2 < 2^15,
+//              = dot2((px, py), w0, dot2((pz, 0), w1, w2 + 4|p|^2)) always
+// (v_dot2_i32_i16: two int16 products plus an int32).  Both forms give the same
+// integer; the short one needs every kept charge within 90 sites of o.  A term
+// is then f[4r2^2] - f[4r1^2] (two LDS reads, byte offsets), times q.
+// The table entries are FP64 values rounded once, so a term's error is at most
+// that of two rounded values; over a 64-term unit the FP32 accumulation
+// dominates, as in the arithmetic form.
+// ---------------------------------------------------------------------------
+constexpr int kTabMax = 4096;   // table entries (16 KB of LDS): cutoffs up to 63
+typedef short v2s16 __attribute__((ext_vector_type(2)));
+struct TabTarget {
+    v2s16 xy, z1, z0;
+    int n4;
+};
+__device__ __forceinline__ v2s16 pk16(int lo, int hi) {
+    return __builtin_bit_cast(v2s16, (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16));
+}
+// n4 = 4 |p|^2 plus the table's LDS byte address, so that the dot products give
+// the entry's address directly
+__device__ __forceinline__ TabTarget tab_target(const float4& p, int ox, int oy, int oz, uint32_t base) {
+    const int x = int(p.x) - ox, y = int(p.y) - oy, z = int(p.z) - oz;
+    return TabTarget{pk16(x, y), pk16(z, 1), pk16(z, 0), int(base) + 4 * (x * x + y * y + z * z)};
+}
+// v_dot2_i32_i16 in its three-operand (VOP3P) form, d = a.lo b.lo + a.hi b.hi + c.
+// (__builtin_amdgcn_sdot2 is selected as the accumulate-in-place form, which
+// costs a copy of a loop-invariant accumulator per use.)  An ALU instruction
+// only: no memory access.
+__device__ __forceinline__ int sdot2(v2s16 a, int b, int c) {
+    int d;
+    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+template <bool kShort>
+__device__ __forceinline__ int tab_r2x4(const TabTarget& p, int w0, int w1, int w2) {
+    const int zq = kShort ? sdot2(p.z1, w1, p.n4) : sdot2(p.z0, w1, w2 + p.n4);
+    return sdot2(p.xy, w0, zq);
+}
+// f at LDS byte address a = base + 4 r^2 (beyond the cutoff, the last entry at lim = base + 4 c^2)
+__device__ __forceinline__ float tab_f(int a, uint32_t lim) {
+    typedef const __attribute__((address_space(3))) float* lds_cfp;
+    return *reinterpret_cast<lds_cfp>(uintptr_t(min(uint32_t(a), lim)));
+}
+
+// kTable: the pair function from the LDS table of f by r^2 (every replica of the
+// launch has one), 4 r^2 from the charge records as in cull_de_kernel (the long
+// form, coordinates from the lattice origin); otherwise 1/r or the erf-screened
+// form evaluated here.  Dynamic LDS: kDeLdsBytes, the table, or the refill's
+// window in the refill workgroup.
+constexpr int kDeLdsBytes = kTabMax * 4;
+static_assert(kDeLdsBytes >= kTailWords * 4, "the refill's window fits de_kernel's dynamic LDS");
+template <bool kGauss, bool kPairs, bool kTable>
 __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict__ Dp) {
     const SimDev& D = Dp[blockIdx.z];     // replica
     __shared__ float4 tile[256];
-    __shared__ uint32_t tail[kTailWords];
+    extern __shared__ float de_lds[];
     if (int(blockIdx.x) >= D.nblk || int(blockIdx.y) >= D.S) {
         if (int(blockIdx.x) == D.nblk && blockIdx.y == 0)
-            mt_refill<kSimBlock, 2>(D, tail, D.ctl->rng_pos + 624ull + D.lookahead);
+            mt_refill<kSimBlock, 2>(D, reinterpret_cast<uint32_t*>(de_lds), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
     }
     const SimCtl& C = *D.ctl;
@@ -497,24 +562,61 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
     const HopTerm ht = hop_term(P.p1, P.p2);
     const uint32_t c2m1 = D.c2m1;
     const float gk = D.gk;
+    // table form: the table into LDS (published by the first tile's barrier)
+    typedef __attribute__((address_space(3))) float* lds_fp;
+    const uint32_t tbase = uint32_t(uintptr_t((lds_fp)de_lds));   // (address-space cast)
+    const uint32_t tlim = tbase + D.rtab_lim;
+    const TabTarget ta1 = tab_target(P.p1, 0, 0, 0, tbase), ta2 = tab_target(P.p2, 0, 0, 0, tbase);
+    if (kTable)
+        for (int k = threadIdx.x; k < D.rtab_n; k += kSimBlock) de_lds[k] = gmem(D.rtab)[k];
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i* rec = reinterpret_cast<const v4i*>(tile);
     long long acc64 = 0;
     for (int base = j0; base < j1; base += 256) {
         const int n = min(256, j1 - base);
         __syncthreads();
-        if (threadIdx.x < n) tile[threadIdx.x] = charge_f4(D, base + threadIdx.x, nd, ne, cur);
+        if (threadIdx.x < n) {
+            const float4 q = charge_f4(D, base + threadIdx.x, nd, ne, cur);
+            if (kTable) {   // the record of cull_de_kernel's table form, from the origin
+                const int qx = int(q.x), qy = int(q.y), qz = int(q.z);
+                const int n4 = 4 * (qx * qx + qy * qy + qz * qz);
+                v4i w;
+                w.x = int((uint32_t(-8 * qx) & 0xffffu) | (uint32_t(-8 * qy) << 16));
+                w.y = int(uint32_t(-8 * qz) & 0xffffu);
+                w.z = n4;
+                w.w = __float_as_int(q.w);
+                reinterpret_cast<v4i*>(tile)[threadIdx.x] = w;
+            } else {
+                tile[threadIdx.x] = q;
+            }
+        }
         __syncthreads();
         int k0 = 0;
         for (; k0 + 32 <= n; k0 += 32) {   // full blocks: unrolled, branch-free
             float acc = 0.f;
 #pragma unroll 8
-            for (int k = k0; k < k0 + 32; ++k)
-                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
+            for (int k = k0; k < k0 + 32; ++k) {
+                if (kTable) {
+                    const v4i w = rec[k];
+                    acc = fmaf(__int_as_float(w.w), tab_f(tab_r2x4<false>(ta2, w.x, w.y, w.z), tlim) -
+                                                        tab_f(tab_r2x4<false>(ta1, w.x, w.y, w.z), tlim), acc);
+                } else {
+                    acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
+                }
+            }
             acc64 += fix_round(acc);
         }
         if (k0 < n) {   // the list's last, partial block
             float acc = 0.f;
-            for (int k = k0; k < n; ++k)
-                acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
+            for (int k = k0; k < n; ++k) {
+                if (kTable) {
+                    const v4i w = rec[k];
+                    acc = fmaf(__int_as_float(w.w), tab_f(tab_r2x4<false>(ta2, w.x, w.y, w.z), tlim) -
+                                                        tab_f(tab_r2x4<false>(ta1, w.x, w.y, w.z), tlim), acc);
+                } else {
+                    acc = fmaf(tile[k].w, de_term<kGauss>(p1, tile[k], ht, c2m1, gk), acc);
+                }
+            }
             acc64 += fix_round(acc);
         }
     }
@@ -877,64 +979,6 @@ __device__ __forceinline__ DecideRec load_rec(const DecideRec* p) {
     DecideRec r;
     __builtin_memcpy(&r, a, sizeof(r));
     return r;
-}
-
-// ---------------------------------------------------------------------------
-// Table form of the culled dE terms.  Every squared distance on the lattice
-// is an integer, so f(r) is a table lookup by r^2: the in-cutoff test, the
-// r = 0 exclusion and (for the erf-screened pair function) erf all fold into
-// the table, whose entries are 0 at r^2 = 0 and at c^2, and indices beyond
-// c^2 are clamped to c^2.  The workgroup copies the table into LDS.
-//
-// 4 r^2 comes out exact from integer dot products.  Coordinates are taken
-// relative to an origin o (the centre of the group's box); a charge q is
-// staged as the record
-//   w0 = (-8 qx, -8 qy), w1 = (-8 qz, 4 |q|^2) as int16 pairs, w2 = 4 |q|^2,
-//   w3 = its charge (FP32 bits)
-// and a target p holds (px, py), (pz, 1), (pz, 0) as int16 pairs and 4 |p|^2:
-//   4 |p - q|^2 = 4|p|^2 + 4|q|^2 - 8 p.q
-//              = dot2((px, py), w0, dot2((pz, 1), w1, 4|p|^2))      if 4|q|^2 < 2^15,
-//              = dot2((px, py), w0, dot2((pz, 0), w1, w2 + 4|p|^2)) always
-// (v_dot2_i32_i16: two int16 products plus an int32).  Both forms give the same
-// integer; the short one needs every kept charge within 90 sites of o.  A term
-// is then f[4r2^2] - f[4r1^2] (two LDS reads, byte offsets), times q.
-// The table entries are FP64 values rounded once, so a term's error is at most
-// that of two rounded values; over a 64-term unit the FP32 accumulation
-// dominates, as in the arithmetic form.
-// ---------------------------------------------------------------------------
-constexpr int kTabMax = 4096;   // table entries (16 KB of LDS): cutoffs up to 63
-typedef short v2s16 __attribute__((ext_vector_type(2)));
-struct TabTarget {
-    v2s16 xy, z1, z0;
-    int n4;
-};
-__device__ __forceinline__ v2s16 pk16(int lo, int hi) {
-    return __builtin_bit_cast(v2s16, (uint32_t(lo) & 0xffffu) | (uint32_t(hi) << 16));
-}
-// n4 = 4 |p|^2 plus the table's LDS byte address, so that the dot products give
-// the entry's address directly
-__device__ __forceinline__ TabTarget tab_target(const float4& p, int ox, int oy, int oz, uint32_t base) {
-    const int x = int(p.x) - ox, y = int(p.y) - oy, z = int(p.z) - oz;
-    return TabTarget{pk16(x, y), pk16(z, 1), pk16(z, 0), int(base) + 4 * (x * x + y * y + z * z)};
-}
-// v_dot2_i32_i16 in its three-operand (VOP3P) form, d = a.lo b.lo + a.hi b.hi + c.
-// (__builtin_amdgcn_sdot2 is selected as the accumulate-in-place form, which
-// costs a copy of a loop-invariant accumulator per use.)  An ALU instruction
-// only: no memory access.
-__device__ __forceinline__ int sdot2(v2s16 a, int b, int c) {
-    int d;
-    asm("v_dot2_i32_i16 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-    return d;
-}
-template <bool kShort>
-__device__ __forceinline__ int tab_r2x4(const TabTarget& p, int w0, int w1, int w2) {
-    const int zq = kShort ? sdot2(p.z1, w1, p.n4) : sdot2(p.z0, w1, w2 + p.n4);
-    return sdot2(p.xy, w0, zq);
-}
-// f at LDS byte address a = base + 4 r^2 (beyond the cutoff, the last entry at lim = base + 4 c^2)
-__device__ __forceinline__ float tab_f(int a, uint32_t lim) {
-    typedef const __attribute__((address_space(3))) float* lds_cfp;
-    return *reinterpret_cast<lds_cfp>(uintptr_t(min(uint32_t(a), lim)));
 }
 
 constexpr int kList = 448;   // LDS charge slots per wave
