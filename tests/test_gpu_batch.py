@@ -22,9 +22,9 @@ BASE = dict(coulomb_carriers=1, electron_percentage=0.0, seed_charges=1.0, volta
             e_source_l_rate=0.0, h_source_l_rate=0.9, h_drain_r_rate=0.9)
 
 
-def build(oracle, gpu, de_path="auto"):
+def build(oracle, gpu, de_path="auto", replicas=REPLICAS):
     worlds, reps = [], []
-    for kw in REPLICAS:
+    for kw in replicas:
         k = dict(BASE)
         if kw.get("simulation_type") == 1:
             k = dict(coulomb_carriers=1)
@@ -64,6 +64,38 @@ def test_batch_matches_single_contexts_and_oracle(gpu, oracle, de_path):
                 assert np.array_equal(bs[k], ss[k]), f"replica {r} step {t}: {k} differs from its single run"
             assert bs["rng_consumed"] == ss["rng_consumed"]
             assert not compare_state(bs, w), f"replica {r} step {t}: differs from the oracle"
+
+
+def run_against_singles(gpu, worlds, batch, steps):
+    paths = batch.de_paths
+    singles = [gpu.Simulation(sp, de_path=paths[r], **st) for r, (_, sp, st) in enumerate(worlds)]
+    for t in range(steps):
+        batch.step(1)
+        for s in singles:
+            s.step(1)
+        for r, (w, sp, _) in enumerate(worlds):
+            w.step(1)
+            bs, ss = batch.state(r), singles[r].state()
+            for k in ("e_site", "h_site", "e_pathlength", "h_pathlength", "rng_window"):
+                assert np.array_equal(bs[k], ss[k]), f"replica {r} step {t}: {k} differs from its single run"
+            assert not compare_state(bs, w), f"replica {r} step {t}: differs from the oracle"
+
+
+@pytest.mark.parametrize("de_path", ["auto", "culled"])
+def test_batch_with_and_without_tables(gpu, oracle, de_path):
+    """A replica whose cutoff (64) is past the pair-function table's reach runs the dE
+    kernels' evaluated form; in a batch it takes a launch of its own (all pairs), and
+    every replica still follows its single context and the oracle."""
+    reps = REPLICAS[:3] + [dict(grid_x=20, grid_y=16, grid_z=4, hole_percentage=0.04,
+                                electrostatic_cutoff=64, random_seed=37)]
+    worlds, batch = build(oracle, gpu, de_path, reps)
+    paths = batch.de_paths
+    print(f"batch dE paths {paths}", flush=True)
+    if de_path == "culled":   # the table-less replica is not culled beside replicas with tables
+        assert paths == ["culled"] * 3 + ["pairs"]
+    else:
+        assert paths == ["pairs"] * 4
+    run_against_singles(gpu, worlds, batch, 30)
 
 
 def test_batch_autotuned_chunks(gpu, oracle):
