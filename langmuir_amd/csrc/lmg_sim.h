@@ -695,21 +695,31 @@ constexpr int kOffPer = 8;
 // The loads of the offsets, issued before the injections (no barrier): the
 // thread's first kOffPer bricks' bperm entries and defect counts, and with cnt
 // the counters into LDS.
+// The loads are global (not flat) and the brick counters stay in registers
+// until cull_offsets stages them in LDS, so that a workgroup barrier between
+// the two (inject_kernel's injections) does not wait for them.
 struct OffsetLoads {
     int bp[kOffPer], dn[kOffPer];
+    int cv[kStageBricks / kOneBlock];   // brick counters b = threadIdx.x + kOneBlock u (cnt staging)
 };
 template <int kThreads, bool kAgent>
 __device__ void cull_offsets_load(const SimDev& D, int* cnt, OffsetLoads& L) {
+    static_assert(kThreads == kOneBlock, "one counter slot per kOneBlock bricks");
     const int nb = D.nbrick, per = (nb + kThreads - 1) / kThreads;
     const int b0 = min(nb, int(threadIdx.x) * per);
 #pragma unroll
     for (int u = 0; u < kOffPer; ++u) {   // unconditional (clamped index): all in flight together
         const int b = min(b0 + u, nb - 1);
-        L.bp[u] = D.bperm[b];
-        L.dn[u] = D.dstart[b + 1] - D.dstart[b];
+        L.bp[u] = gmem(D.bperm)[b];
+        L.dn[u] = gmem(D.dstart)[b + 1] - gmem(D.dstart)[b];
     }
-    if (cnt)
-        for (int b = threadIdx.x; b < nb; b += kThreads) cnt[b] = kAgent ? load_agent(&D.scnt[b]) : D.scnt[b];
+    if (cnt) {
+#pragma unroll
+        for (int u = 0; u < kStageBricks / kThreads; ++u) {
+            const int b = int(threadIdx.x) + kThreads * u;
+            L.cv[u] = b < nb ? (kAgent ? load_agent(&D.scnt[b]) : gmem(D.scnt)[b]) : 0;
+        }
+    }
 }
 template <int kThreads, bool kAgent>
 __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, const int* inj_b, int inj_n,
@@ -721,6 +731,11 @@ __device__ void cull_offsets(const SimDev& D, int* sst, int* tst, int* wsum, con
     const int* bp = L.bp;
     const int* dn = L.dn;
     if (cnt) {
+#pragma unroll
+        for (int u = 0; u < kStageBricks / kThreads; ++u) {
+            const int b = int(threadIdx.x) + kThreads * u;
+            if (b < nb) cnt[b] = L.cv[u];
+        }
         __syncthreads();   // the staged counters
         for (int k = threadIdx.x; k < inj_n; k += kThreads) atomicAdd(&cnt[inj_b[k]], 1);
         __syncthreads();
@@ -2066,10 +2081,10 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
     unsigned long long dl[kDangerSlots];
     if (mode == 0) {
         if (D.cull) cull_offsets_load<kOneBlock, false>(D, cnt, ol);
-        if (threadIdx.x == 0) {
-            nd = C.danger_n;
+        if (threadIdx.x == 0) {   // (global loads: the barrier below does not wait for them)
+            nd = gmem(D.ctl)->danger_n;
 #pragma unroll
-            for (int k = 0; k < kDangerSlots; ++k) dl[k] = C.danger[k];
+            for (int k = 0; k < kDangerSlots; ++k) dl[k] = gmem(D.ctl)->danger[k];
         }
     }
     __syncthreads();
