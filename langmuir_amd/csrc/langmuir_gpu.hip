@@ -857,12 +857,12 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         return st;
     if (D.cull) {
         const int nb = D.nbrick;
-        int *bperm = nullptr, *dstart = nullptr;
+        int *bperm = nullptr, *brank = nullptr, *dstart = nullptr;
         float4* dsrc = nullptr;
         if ((st = sim_alloc(c, D.bmask, size_t(nb) * 32)) || (st = sim_alloc(c, D.scnt, nb)) ||
             (st = sim_alloc(c, D.sstart, nb + 1)) || (st = sim_alloc(c, D.tstart, nb)) ||
             (st = sim_alloc(c, D.e_who, d.V)) || (st = sim_alloc(c, D.h_who, d.V)) ||
-            (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
+            (st = sim_alloc(c, bperm, nb)) || (st = sim_alloc(c, brank, nb)) || (st = sim_alloc(c, dstart, nb + 1)) ||
             (st = sim_alloc(c, dsrc, D.n_d_eff)) || (st = sim_alloc(c, D.src, size_t(cap) + D.n_d_eff + 3 * size_t(nb))) ||
             (st = sim_alloc(c, D.torder, cap)) || (st = sim_alloc(c, D.tgrp, D.ngrp)) ||
             (st = sim_alloc(c, D.items, 4 * size_t(c->cull_quads_alloc))) || (st = sim_alloc(c, D.nitems, 1)) ||
@@ -936,6 +936,9 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         std::vector<int> perm(nb);
         for (int r = 0; r < nb; ++r) perm[r] = key[r].second;
         HIPCHK(c, hipMemcpy(bperm, perm.data(), size_t(nb) * 4, hipMemcpyHostToDevice));
+        std::vector<int> rank(nb);
+        for (int r = 0; r < nb; ++r) rank[perm[r]] = r;
+        HIPCHK(c, hipMemcpy(brank, rank.data(), size_t(nb) * 4, hipMemcpyHostToDevice));
         // charged defects by brick, then site, then list order
         std::vector<std::pair<std::pair<int, int>, int>> dk(D.n_d_eff);
         for (int k = 0; k < D.n_d_eff; ++k) {
@@ -957,6 +960,7 @@ lmg_status lmg_sim_load(lmg_ctx* c, const lmg_sim_config* cfg, const lmg_sim_sta
         HIPCHK(c, hipMemcpy(dstart, ds.data(), ds.size() * 4, hipMemcpyHostToDevice));
         if (D.n_d_eff > 0) HIPCHK(c, hipMemcpy(dsrc, dv.data(), dv.size() * sizeof(float4), hipMemcpyHostToDevice));
         D.bperm = bperm;
+        D.brank = brank;
         D.dstart = dstart;
         D.dsrc = dsrc;
     }

@@ -93,8 +93,10 @@ struct SimDev {
     int* e_who;                   // [V] list index of the electron on a site (valid where its bit is set)
     int* h_who;                   // [V] the same for holes
     int* sstart;                  // [nbrick + 1] charge list offsets
-    int* tstart;                  // [nbrick] first target slot of each brick (bricks in bperm order)
+    int* tstart;                  // [nbrick] first target slot of each brick (bricks in bperm order;
+                                  // written only for more than kEmitScanBricks bricks)
     const int* bperm;             // brick order of the target groups (static, set at load)
+    const int* brank;             // [nbrick] each brick's rank in bperm order (bperm's inverse)
     const int* dstart;            // [nbrick + 1] static charged-defect list offsets
     const float4* dsrc;           // static charged defects, by brick then site
     float4* src;                  // charge lists of the bricks
@@ -692,6 +694,11 @@ __device__ __forceinline__ void cull_mark(const SimDev& D, bool isE, int li, int
 // scope (needed when this workgroup marked them).
 constexpr int kStageBricks = 8192;
 constexpr int kOffPer = 8;
+// Lattices of up to kEmitScanBricks bricks take their offsets in cull_emit_kernel
+// instead: each of its brick workgroups sums the counters before its own bricks
+// (kEmitScanBricks / kSimBlock counters and ranks per thread, all loads in
+// flight together), so the serial scan leaves inject_kernel's one workgroup.
+constexpr int kEmitScanBricks = 2048;
 // The loads of the offsets, issued before the injections (no barrier): the
 // thread's first kOffPer bricks' bperm entries and defect counts, and with cnt
 // the counters into LDS.
@@ -844,7 +851,9 @@ __device__ __forceinline__ int2 cull_workgroup(const int2* __restrict__ map) {
     return map ? map[blockIdx.x] : make_int2(0, int(blockIdx.x));
 }
 
-// per replica ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items)
+// per replica ceil(nbrick / 4) + nblk + 1 workgroups (the last one: the work items);
+// a brick workgroup's four waves copy four bricks' carriers and defects into
+// their charge lists and their carriers into the target order
 __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __restrict__ Dp,
                                                               const int2* __restrict__ map) {
     const int2 wg = cull_workgroup(map);
@@ -880,16 +889,69 @@ __global__ __launch_bounds__(kSimBlock) void cull_emit_kernel(const SimDev* __re
         if (threadIdx.x == 0) D.blk_draw[wx - nbw] = cnt;
         return;
     }
-    const int b = wx * 4 + (threadIdx.x >> 6);
-    if (b >= D.nbrick) return;
-    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = wx * 4 + wid, nb = D.nbrick;
+    int base, end, t0;
+    if (nb <= kEmitScanBricks) {
+        // this workgroup's list offsets: the carriers of the bricks before each
+        // of its four in linear order (charge lists, after the defects of
+        // dstart) and in bperm order (targets), from the counters that
+        // tick_kernel and inject_kernel marked
+        constexpr int kPer = kEmitScanBricks / kSimBlock;
+        const __attribute__((address_space(1))) int* cg = gmem(D.scnt);
+        const __attribute__((address_space(1))) int* rg = gmem(D.brank);
+        int rk[4], c[kPer], r[kPer];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) rk[w] = rg[min(wx * 4 + w, nb - 1)];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {   // clamped indices: every load unconditional
+            const int bb = int(threadIdx.x) + kSimBlock * u;
+            c[u] = cg[min(bb, nb - 1)];
+            r[u] = rg[min(bb, nb - 1)];
+            if (bb >= nb) c[u] = 0;
+        }
+        const int nmark = cg[min(b, nb - 1)];
+        const int d0 = gmem(D.dstart)[min(b, nb - 1)], d1 = gmem(D.dstart)[min(b, nb - 1) + 1];
+        int sum[8];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            sum[w] = sum[4 + w] = 0;
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int bb = int(threadIdx.x) + kSimBlock * u;
+                sum[w] += bb < wx * 4 + w ? c[u] : 0;
+                sum[4 + w] += r[u] < rk[w] ? c[u] : 0;
+            }
+        }
+        __shared__ int part[4][8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = wave_total(wave_incl_sum(sum[k]));
+            if (lane == 0) part[wid][k] = t;
+        }
+        __syncthreads();
+        if (b >= nb) return;
+        const int lin = part[0][wid] + part[1][wid] + part[2][wid] + part[3][wid];
+        t0 = part[0][4 + wid] + part[1][4 + wid] + part[2][4 + wid] + part[3][4 + wid];
+        base = d0 + lin;
+        end = d1 + lin + nmark;
+        if (lane == 0) {
+            gst(D.sstart)[b] = base;   // cull_de_kernel's list bounds
+            if (b == nb - 1) gst(D.sstart)[nb] = end;
+        }
+    } else {
+        if (b >= nb) return;
+        base = sst[b];
+        end = sst[b + 1];
+        t0 = tst[b];
+    }
     const uint32_t v = lane < 32 ? D.bmask[b * 32 + lane] : 0u;
     const int cnt = __popc(v);
     const int incl = wave_incl_sum(cnt);
     const int neh = __builtin_amdgcn_readlane(incl, 31);
     const int nmark = D.scnt[b];
     if (lane == 0 && neh != nmark) set_error(D, kErrSite);   // two of one species on a site
-    const int base = sst[b], end = sst[b + 1], t0 = tst[b], ne = D.ctl->n_e;
+    const int ne = D.ctl->n_e;
     const int bx = b % D.nbx, byz = b / D.nbx, by = byz % D.nby, bz = byz / D.nby;
     const bool isE = lane < 16;
     const float q = isE ? -1.f : 1.f;
@@ -2043,7 +2105,8 @@ __device__ unsigned long long propose_serial(const SimDev& D, int i0, int ne, in
 // mode 1: refill the random stream after a state load
 // Then the next step's set-up: on the culled path the brick counters (mode 1
 // marks every carrier; in mode 0 tick_kernel and the injections marked them)
-// become list offsets; the proposal draws are checked for a Boost rejection
+// become list offsets here for lattices of more than kEmitScanBricks bricks
+// (smaller ones in cull_emit_kernel); the proposal draws are checked for a Boost rejection
 // (first_bad), from which the proposals are made serially here -- all of them
 // when some site has fewer than two neighbours (one-entry lists draw nothing).
 // Leaves the new lists' sizes, the proposal start rng_step0, first_bad, the
@@ -2075,12 +2138,14 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
     }
     // after a step: the offsets' loads and the listed engine outputs before the
     // injections (neither changes there)
-    int* cnt = D.cull && D.nbrick <= kStageBricks ? sh.cnt : nullptr;
+    // (lattices of up to kEmitScanBricks bricks: cull_emit_kernel's workgroups take their own offsets)
+    const bool offs = D.cull && D.nbrick > kEmitScanBricks;
+    int* cnt = offs && D.nbrick <= kStageBricks ? sh.cnt : nullptr;
     OffsetLoads ol;
     int nd = 0;
     unsigned long long dl[kDangerSlots];
     if (mode == 0) {
-        if (D.cull) cull_offsets_load<kOneBlock, false>(D, cnt, ol);
+        if (offs) cull_offsets_load<kOneBlock, false>(D, cnt, ol);
         if (threadIdx.x == 0) {   // (global loads: the barrier below does not wait for them)
             nd = gmem(D.ctl)->danger_n;
 #pragma unroll
@@ -2148,11 +2213,13 @@ __global__ __launch_bounds__(kOneBlock) void inject_kernel(const SimDev* __restr
                 const int li = isE ? i : i - ne;
                 cull_mark(D, isE, li, isE ? D.e_site[cur][li] : D.h_site[cur][li]);
             }
-            __syncthreads();
-            cull_offsets_load<kOneBlock, true>(D, cnt, ol);
-            cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0, cnt, ol);
+            if (offs) {
+                __syncthreads();
+                cull_offsets_load<kOneBlock, true>(D, cnt, ol);
+                cull_offsets<kOneBlock, true>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, 0, cnt, ol);
+            }
         } else {   // tick_kernel's marks plus this step's injected carriers
-            cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n, cnt, ol);
+            if (offs) cull_offsets<kOneBlock, false>(D, D.sstart, D.tstart, sh.wsum, sh.inj_b, sh.inj_n, cnt, ol);
             for (int k = threadIdx.x; k < sh.inj_n; k += blockDim.x) atomicAdd(&D.scnt[sh.inj_b[k]], 1);
         }
     }
