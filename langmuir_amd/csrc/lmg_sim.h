@@ -511,6 +511,21 @@ __device__ __forceinline__ int tab_r2x4(const TabTarget& p, int w0, int w1, int 
     const int zq = kShort ? sdot2(p.z1, w1, p.n4) : sdot2(p.z0, w1, w2 + p.n4);
     return sdot2(p.xy, w0, zq);
 }
+// A staged record (w0, w1, w2, charge bits) read from LDS in narrow pieces: w0 and w1
+// as 64 bits and the charge as 32 (the short form has no use for w2), or two
+// 64-bit halves.  Every lane reads the same record; among the table reads
+// (about 2.4 CU cycles per 32-bit read at this spread of addresses) such a
+// 128-bit broadcast read costs about 6 CU cycles on gfx950 and the 64 + 32-bit
+// pair about 3.
+typedef int rec_v4i __attribute__((ext_vector_type(4)));
+template <bool kShort>
+__device__ __forceinline__ rec_v4i rec_load(const rec_v4i* L, int t) {
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    const v2i a = reinterpret_cast<const v2i*>(L + t)[0];
+    if (kShort) return rec_v4i{a.x, a.y, 0, reinterpret_cast<const int*>(L + t)[3]};
+    const v2i b = reinterpret_cast<const v2i*>(L + t)[1];
+    return rec_v4i{a.x, a.y, b.x, b.y};
+}
 // f at LDS byte address a = base + 4 r^2 (beyond the cutoff, the last entry at lim = base + 4 c^2)
 __device__ __forceinline__ float tab_f(int a, uint32_t lim) {
     typedef const __attribute__((address_space(3))) float* lds_cfp;
@@ -599,7 +614,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
 #pragma unroll 8
             for (int k = k0; k < k0 + 32; ++k) {
                 if (kTable) {
-                    const v4i w = rec[k];
+                    const v4i w = rec_load<false>(rec, k);
                     acc = fmaf(__int_as_float(w.w), tab_f(tab_r2x4<false>(ta2, w.x, w.y, w.z), tlim) -
                                                         tab_f(tab_r2x4<false>(ta1, w.x, w.y, w.z), tlim), acc);
                 } else {
@@ -612,7 +627,7 @@ __global__ __launch_bounds__(kSimBlock) void de_kernel(const SimDev* __restrict_
             float acc = 0.f;
             for (int k = k0; k < n; ++k) {
                 if (kTable) {
-                    const v4i w = rec[k];
+                    const v4i w = rec_load<false>(rec, k);
                     acc = fmaf(__int_as_float(w.w), tab_f(tab_r2x4<false>(ta2, w.x, w.y, w.z), tlim) -
                                                         tab_f(tab_r2x4<false>(ta1, w.x, w.y, w.z), tlim), acc);
                 } else {
@@ -1058,7 +1073,8 @@ __device__ __forceinline__ DecideRec load_rec(const DecideRec* p) {
     return r;
 }
 
-constexpr int kList = 448;   // LDS charge slots per wave
+constexpr int kList = 256;   // LDS charge slots per wave (24 KB of LDS per workgroup with a
+                             // cutoff-30 table: six workgroups per CU)
 constexpr int kGather = 8;   // units whose loads a wave keeps in flight together
 constexpr int kMaxCullS = 32;   // chunks per target group
 constexpr int kUnit = 64;    // summation unit: a 64-slot piece of a brick's list
@@ -1091,8 +1107,10 @@ inline size_t cull_lds_bytes(int rtab_n) {
 // and the units a group skips hold only terms that are exactly 0 for its
 // targets; integer addition is exact, so each target's total is the same
 // whatever its group, chunk count or order.
+// (table form: registers for six waves per SIMD, 80 VGPRs, which the LDS of six
+// workgroups allows; more waves hide more of the chain and the table reads)
 template <bool kGauss, bool kTable>
-__global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __restrict__ Dp,
+__global__ __launch_bounds__(kSimBlock, kTable ? 6 : 4) void cull_de_kernel(const SimDev* __restrict__ Dp,
                                                             const int2* __restrict__ map) {
     const int2 wg = cull_workgroup(map);
     const SimDev& D = Dp[wg.x];
@@ -1108,7 +1126,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
         // The random-stream refill.  Its CU is LDS bound (the pair sums), where
         // the 227-word rounds (three LDS reads a word) run faster than the wide
         // ones; its serial rounds issue first.
-        static_assert(3 * kList * sizeof(float4) >= kTailWords * sizeof(uint32_t), "refill tail in lists 0-2");
+        static_assert(4 * kList * sizeof(float4) >= kTailWords * sizeof(uint32_t), "refill window in the lists");
         __builtin_amdgcn_s_setprio(3);
         mt_refill<kSimBlock, 0>(D, reinterpret_cast<uint32_t*>(&lists[0][0]), D.ctl->rng_pos + 624ull + D.lookahead);
         return;
@@ -1212,7 +1230,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             v4i w[4];
             float d[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) w[j] = L[t + j];
+            for (int j = 0; j < 4; ++j) w[j] = rec_load<kS>(L, t + j);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 d[j] = tab_f(tab_r2x4<kS>(ta2, w[j].x, w[j].y, w[j].z), tlim) -
@@ -1221,7 +1239,7 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
             for (int j = 0; j < 4; ++j) acc = fmaf(__int_as_float(w[j].w), d[j], acc);
         }
         for (; t < e; ++t) {
-            const v4i w = L[t];
+            const v4i w = rec_load<kS>(L, t);
             const float d = tab_f(tab_r2x4<kS>(ta2, w.x, w.y, w.z), tlim) -
                             tab_f(tab_r2x4<kS>(ta1, w.x, w.y, w.z), tlim);
             acc = fmaf(__int_as_float(w.w), d, acc);
@@ -1368,12 +1386,12 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
     if (ncand <= 2 * kSimBlock) {   // the same for the four waves (one group)
         // The workgroup tests the candidates once, two per thread, into a table of
         // list lengths, list starts and positions (slots of the candidates before),
-        // in wave 3's list, which the gathers only overwrite after a barrier.
-        int* tab_n = reinterpret_cast<int*>(&lists[3][0]);
+        // in the lists of waves 2 and 3, which the gathers only overwrite after a barrier.
+        int* tab_n = reinterpret_cast<int*>(&lists[2][0]);
         int* tab_k = tab_n + 2 * kSimBlock;
         int* tab_p = tab_k + 2 * kSimBlock;
         int* wtot = tab_p + 2 * kSimBlock;   // the waves' totals
-        static_assert(6 * kSimBlock + 4 <= 4 * kList, "candidate table in one wave's list");
+        static_assert(6 * kSimBlock + 4 <= 8 * kList, "candidate table in two waves' lists");
         int ka, kb;
         const int c0 = 2 * int(threadIdx.x);
         const int na = brick(c0, ka), nb = brick(c0 + 1, kb);
@@ -1445,13 +1463,14 @@ __global__ __launch_bounds__(kSimBlock) void cull_de_kernel(const SimDev* __rest
                     }
 #pragma unroll
                     for (int j = 0; j < kGather; ++j) {
-                        if (u + j >= nb64) break;
-                        const bool k = in[j] && near(a[j]);
-                        const uint64_t m = __ballot(k);
-                        if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = stage(a[j]);
-                        const int n = __popcll(m);
-                        uends[wv][u0 + u + j] = (o << 16) | (o + n);
-                        o += n;
+                        if (u + j < nb64) {   // (no early exit: the unrolled loads stay in registers)
+                            const bool k = in[j] && near(a[j]);
+                            const uint64_t m = __ballot(k);
+                            if (k) dst[o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0))] = stage(a[j]);
+                            const int n = __popcll(m);
+                            uends[wv][u0 + u + j] = (o << 16) | (o + n);
+                            o += n;
+                        }
                     }
                 }
             }
