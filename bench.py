@@ -409,7 +409,9 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
     gate_err, used = sur.last_gate()
     full_err = float(np.linalg.norm(d_out.download(np.float32, sur.V) - snaps.targets(n_snap - 1))
                      / np.linalg.norm(snaps.targets(n_snap - 1)))
-    ms_infer = timed(lambda: sur.infer_dev(d_occ.ptr, 1, d_out.ptr), reps)
+    ms_infer = timed(lambda: sur.map_dev(d_s.ptr, d_q.ptr, len(sites), d_out.ptr, 0.0, wait=False), 5)   # gate fails
+    _, fb_used = sur.last_gate()
+    ms_fallback, ms_infer = ms_infer, timed(lambda: sur.infer_dev(d_occ.ptr, 1, d_out.ptr), reps)
     # the eight held-out snapshots as one batch of gated maps
     nb = n_snap - n_train
     lists = []
@@ -452,7 +454,7 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
         f"{hist[0]:.2e} -> {hist[-1]:.2e}); held-out rel RMS max {max(errs):.2e}; gated map {ms_gated * 1e3:.1f} us "
         f"(gate {gate_err:.2e}, exact used {used}), infer {ms_infer * 1e3:.1f} us; exact map culled "
         f"{ms_exact * 1e3:.1f} us -> {ms_exact / ms_gated:.1f}x, all pairs {ms_pairs * 1e3:.1f} us -> "
-        f"{ms_pairs / ms_gated:.1f}x")
+        f"{ms_pairs / ms_gated:.1f}x; a failed gate (exact fallback) {ms_fallback * 1e3:.1f} us")
     return {
         "maps_per_s": round(1e3 / ms_gated, 2), "exact_maps_per_s": round(1e3 / ms_exact, 2),
         "speedup_vs_exact": round(ms_exact / ms_gated, 2),
@@ -462,6 +464,7 @@ def _bench_surrogate(workload, n_snap, stride, iters, reps):
         "exact_map_all_pairs_us": round(ms_pairs * 1e3, 2),
         "rel_rms_heldout_max": max(errs), "gate_error": gate_err, "gate_sites": 1024, "used_exact": used,
         "rel_rms_gated_map": full_err,
+        "failed_gate_map_us": round(ms_fallback * 1e3, 2), "failed_gate_used_exact": fb_used,
         "batch": {"maps": nb, "gated_batch_us": round(ms_batch * 1e3, 2), "us_per_map": round(us_per_map, 2),
                   "maps_per_s": round(1e6 / us_per_map, 1),
                   "speedup_vs_exact": round(ms_exact * 1e3 / us_per_map, 2),

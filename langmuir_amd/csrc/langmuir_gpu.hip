@@ -203,18 +203,13 @@ bool map_culled(const lmg_ctx* c, int nc, int flags) {
     if ((flags & LMG_NO_CUTOFF) || nc == 0) return false;
     if (flags & LMG_MAP_CULLED) return true;
     if (flags & LMG_MAP_PAIRS) return false;
-    auto frac = [&](int n) { return std::min(1.0, double(2 * c->p.cutoff + 12) / n); };
-    return frac(c->d.X) * frac(c->d.Y) * frac(c->d.Z) < 0.5 && double(c->d.V) * nc >= 1e8;
+    return map_cull_pays(c->d, c->p.cutoff, nc);
 }
 
 lmg_status map_cull_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, int nc, int flags, float* d_out,
                          const int32_t* d_run) {
     lmg_status st;
-    MapBricks m;
-    m.nbx = (c->d.X + kMapBrick - 1) / kMapBrick;
-    m.nby = (c->d.Y + kMapBrick - 1) / kMapBrick;
-    m.nbz = (c->d.Z + kMapBrick - 1) / kMapBrick;
-    m.nb = m.nbx * m.nby * m.nbz;
+    MapBricks m = map_bricks_shape(c->d);
     const size_t cap = std::max<size_t>(nc, 1);
     void* before = c->b_mcnt.p;
     if ((st = ensure(c, c->b_mcnt, size_t(m.nb) * 4)) || (st = ensure(c, c->b_mstart, size_t(m.nb + 1) * 4)) ||
@@ -228,32 +223,8 @@ lmg_status map_cull_core(lmg_ctx* c, const int32_t* d_cs, const int32_t* d_cq, i
     m.tmp = static_cast<float4*>(c->b_mtmp.p);
     m.tmpi = static_cast<int*>(c->b_mtmpi.p);
     m.src = static_cast<float4*>(c->b_msrc.p);
-    const dim3 cg((nc + 255) / 256);
-    hipLaunchKernelGGL(mapc_count_kernel, cg, dim3(256), 0, c->stream, c->d, m, d_cs, nc, d_run);
-    hipLaunchKernelGGL(mapc_scan_kernel, dim3(1), dim3(1024), 0, c->stream, m, d_run);
-    hipLaunchKernelGGL(mapc_scatter_kernel, cg, dim3(256), 0, c->stream, c->d, m, d_cs, d_cq, nc, d_run);
-    hipLaunchKernelGGL(mapc_sort_kernel, dim3((m.nb + 3) / 4), dim3(256), 0, c->stream, m, d_run);
-    const int flat = c->d.Z < 4 ? 1 : 0;
-    const int zc = c->d.Z >= 16 ? 4 : 1;   // sites per lane along z
-    int ts, tz;
-    map_tile_shape(zc, flat, ts, tz);
-    const long long tiles = (long long)((c->d.X + ts - 1) / ts) * ((c->d.Y + ts - 1) / ts) * ((c->d.Z + tz - 1) / tz);
-    const dim3 mg(unsigned((tiles + 3) / 4));
-    const uint32_t c2m1 = cutoff_bits(c->p.cutoff, false);
-    const float gk = gauss_k(c->p.gaussian_sigma);
-    const float pref = float(c->p.prefactor);
-    const bool gauss = flags & LMG_GAUSS;
-#define LMG_MAP_CULL(G, Z) \
-    hipLaunchKernelGGL((map_cull_kernel<G, Z>), mg, dim3(256), 0, c->stream, c->d, m, c->p.cutoff, c2m1, gk, pref, \
-                       flat, d_run, d_out)
-    if (zc == 4) {
-        if (gauss) LMG_MAP_CULL(true, 4);
-        else LMG_MAP_CULL(false, 4);
-    } else {
-        if (gauss) LMG_MAP_CULL(true, 1);
-        else LMG_MAP_CULL(false, 1);
-    }
-#undef LMG_MAP_CULL
+    enqueue_map_cull(c->stream, c->d, m, c->p.cutoff, gauss_k(c->p.gaussian_sigma), float(c->p.prefactor),
+                     flags & LMG_GAUSS, d_cs, d_cq, nc, 1, MapBatch{nullptr, d_run, nullptr}, d_out);
     return launch_check(c, "map_cull_kernel");
 }
 

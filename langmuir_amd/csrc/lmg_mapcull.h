@@ -20,6 +20,7 @@
 // brick has r >= c from every site of the tile, so its term is 0.
 #pragma once
 
+#include <algorithm>
 #include <climits>
 
 #include "lmg_pairsum.h"
@@ -45,10 +46,41 @@ __device__ __forceinline__ int map_brick(const MapBricks& m, int x, int y, int z
     return (x >> 3) + m.nbx * ((y >> 3) + m.nby * (z >> 3));
 }
 
-__global__ void mapc_count_kernel(Dims d, MapBricks m, const int* __restrict__ site, int n,
-                                  const int32_t* __restrict__ run) {
+// A batch of maps: map b = blockIdx.y owns charges [off[b], off[b + 1]) of the
+// lists (with off NULL, a single map of n charges), counts and offsets
+// [b nb, (b + 1) nb) and [b (nb + 1), ...), and runs only if run[b] != 0 and
+// (mode NULL or) mode[b] != 0 -- both decided on the device or uploaded
+// beforehand.  On return m and n describe map b alone, whose charges start at
+// entry base of the lists.
+struct MapBatch {
+    const int* off;
+    const int32_t* run;
+    const int32_t* mode;
+};
+
+__device__ __forceinline__ bool map_slice(const MapBatch& mb, MapBricks& m, int& base, int& n) {
+    const int b = blockIdx.y;
+    if ((mb.run && mb.run[b] == 0) || (mb.mode && mb.mode[b] == 0)) return false;
+    base = 0;
+    if (mb.off) {
+        base = mb.off[b];
+        n = mb.off[b + 1] - base;
+    }
+    m.cnt += b * m.nb;
+    m.start += b * (m.nb + 1);
+    m.slot += base;
+    m.tmp += base;
+    m.tmpi += base;
+    m.src += base;
+    return true;
+}
+
+__global__ void mapc_count_kernel(Dims d, MapBricks m, const int* __restrict__ site, int n, MapBatch mb) {
+    int base;
+    if (!map_slice(mb, m, base, n)) return;
+    site += base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (run && *run == 0)) return;
+    if (i >= n) return;
     const int s = site[i];
     if (s < 0 || s >= d.V) {   // not a lattice site: left out (the host entry points reject it)
         m.slot[i] = make_int2(-1, 0);
@@ -62,8 +94,9 @@ __global__ void mapc_count_kernel(Dims d, MapBricks m, const int* __restrict__ s
 
 // one workgroup of 1024: list offsets (thread t scans a contiguous range of
 // bricks, its first eight counts loaded together)
-__global__ __launch_bounds__(1024) void mapc_scan_kernel(MapBricks m, const int32_t* __restrict__ run) {
-    if (run && *run == 0) return;
+__global__ __launch_bounds__(1024) void mapc_scan_kernel(MapBricks m, MapBatch mb) {
+    int base, n = 0;
+    if (!map_slice(mb, m, base, n)) return;
     __shared__ int wsum[16];
     constexpr int kPer = 8;
     const int nb = m.nb, per = (nb + 1023) / 1024;
@@ -101,9 +134,13 @@ __global__ __launch_bounds__(1024) void mapc_scan_kernel(MapBricks m, const int3
 }
 
 __global__ void mapc_scatter_kernel(Dims d, MapBricks m, const int* __restrict__ site, const int* __restrict__ q,
-                                    int n, const int32_t* __restrict__ run) {
+                                    int n, MapBatch mb) {
+    int base;
+    if (!map_slice(mb, m, base, n)) return;
+    site += base;
+    if (q) q += base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (run && *run == 0)) return;
+    if (i >= n) return;
     const int2 bs = m.slot[i];
     if (bs.x < 0) return;
     const int k = m.start[bs.x] + bs.y;
@@ -114,8 +151,9 @@ __global__ void mapc_scatter_kernel(Dims d, MapBricks m, const int* __restrict__
 }
 
 // grid = ceil(nb / 4), one wave per brick
-__global__ __launch_bounds__(256) void mapc_sort_kernel(MapBricks m, const int32_t* __restrict__ run) {
-    if (run && *run == 0) return;
+__global__ __launch_bounds__(256) void mapc_sort_kernel(MapBricks m, MapBatch mb) {
+    int base, nmap = 0;
+    if (!map_slice(mb, m, base, nmap)) return;
     const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= m.nb) return;
     const int lane = threadIdx.x & 63;
@@ -146,9 +184,12 @@ __host__ __device__ inline void map_tile_shape(int kzc, int flat, int& tsx, int&
 // grid = ceil(tiles / 4), one wave per tile
 template <bool kGauss, int kZc>
 __global__ __launch_bounds__(256) void map_cull_kernel(Dims d, MapBricks m, int cut, uint32_t c2m1, float gk,
-                                                       float pref, int flat, const int32_t* __restrict__ run,
-                                                       float* __restrict__ out) {
-    if (run && *run == 0) return;
+                                                       float pref, int flat, MapBatch mb, float* __restrict__ out) {
+    {
+        int base, n = 0;
+        if (!map_slice(mb, m, base, n)) return;
+        out += (long long)blockIdx.y * d.V;
+    }
     __shared__ float4 lists[4][kMapList];
     __shared__ int uends[4][kMapList];   // a unit holds at least one slot
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -268,6 +309,58 @@ __global__ __launch_bounds__(256) void map_cull_kernel(Dims d, MapBricks m, int 
         for (int k = 0; k < kZc; ++k)
             if (zb + k < d.Z) out[encode(d, x, y, zb + k)] = pref * (sum[k] - comp[k]);
     }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers, shared by the context's map and the surrogate's exact fallback
+// ---------------------------------------------------------------------------
+
+// The culled map pays when the cutoff box of a 64-site tile covers well under
+// the lattice and there is enough work to matter (V x n >= 1e8).
+inline bool map_cull_pays(const Dims& d, int cutoff, long long nc) {
+    auto frac = [&](int n) { return std::min(1.0, double(2 * cutoff + 12) / n); };
+    return nc > 0 && frac(d.X) * frac(d.Y) * frac(d.Z) < 0.5 && double(d.V) * double(nc) >= 1e8;
+}
+
+inline MapBricks map_bricks_shape(const Dims& d) {
+    MapBricks m{};
+    m.nbx = (d.X + kMapBrick - 1) / kMapBrick;
+    m.nby = (d.Y + kMapBrick - 1) / kMapBrick;
+    m.nbz = (d.Z + kMapBrick - 1) / kMapBrick;
+    m.nb = m.nbx * m.nby * m.nbz;
+    return m;
+}
+
+// Every launch of `batch` culled maps on stream st (map b written at
+// d_out + b V).  m's arrays hold batch x nb (cnt, zero on entry and again on
+// exit), batch x (nb + 1) (start) and one entry per charge of the lists (the
+// rest).  mb as for map_slice; with mb.off NULL, batch is 1 and the list has nc
+// charges, else nc is the longest list.
+inline void enqueue_map_cull(hipStream_t st, const Dims& d, const MapBricks& m, int cutoff, float gk, float pref,
+                             bool gauss, const int32_t* d_cs, const int32_t* d_cq, int nc, int batch,
+                             const MapBatch& mb, float* d_out) {
+    const dim3 cg((nc + 255) / 256, batch);
+    hipLaunchKernelGGL(mapc_count_kernel, cg, dim3(256), 0, st, d, m, d_cs, nc, mb);
+    hipLaunchKernelGGL(mapc_scan_kernel, dim3(1, batch), dim3(1024), 0, st, m, mb);
+    hipLaunchKernelGGL(mapc_scatter_kernel, cg, dim3(256), 0, st, d, m, d_cs, d_cq, nc, mb);
+    hipLaunchKernelGGL(mapc_sort_kernel, dim3((m.nb + 3) / 4, batch), dim3(256), 0, st, m, mb);
+    const int flat = d.Z < 4 ? 1 : 0;
+    const int zc = d.Z >= 16 ? 4 : 1;   // sites per lane along z
+    int ts, tz;
+    map_tile_shape(zc, flat, ts, tz);
+    const long long tiles = (long long)((d.X + ts - 1) / ts) * ((d.Y + ts - 1) / ts) * ((d.Z + tz - 1) / tz);
+    const dim3 mg(unsigned((tiles + 3) / 4), batch);
+    const uint32_t c2m1 = cutoff_bits(cutoff, false);
+#define LMG_MAP_CULL(G, Z) \
+    hipLaunchKernelGGL((map_cull_kernel<G, Z>), mg, dim3(256), 0, st, d, m, cutoff, c2m1, gk, pref, flat, mb, d_out)
+    if (zc == 4) {
+        if (gauss) LMG_MAP_CULL(true, 4);
+        else LMG_MAP_CULL(false, 4);
+    } else {
+        if (gauss) LMG_MAP_CULL(true, 1);
+        else LMG_MAP_CULL(false, 1);
+    }
+#undef LMG_MAP_CULL
 }
 
 }  // namespace

@@ -201,14 +201,15 @@ __global__ __launch_bounds__(kBlock) void map_kernel(Dims d, const float4* __res
 }
 
 // A batch of conditional maps: grid z = Z * batch; map b = blockIdx.z / Z of
-// charges [off[b], off[b+1]) into out + b * V when run[b] is set.
+// charges [off[b], off[b+1]) into out + b * V when run[b] is set and skip
+// (if not NULL) has skip[b] == 0.
 template <bool kGauss>
 __global__ __launch_bounds__(kBlock) void map_batch_kernel(Dims d, const float4* __restrict__ chg,
                                                            const int* __restrict__ off, uint32_t c2m1, float gk,
                                                            float pref, int tx_n, const int32_t* __restrict__ run,
-                                                           float* __restrict__ out) {
+                                                           const int32_t* __restrict__ skip, float* __restrict__ out) {
     const int b = blockIdx.z / d.Z;
-    if (run[b] == 0) return;
+    if (run[b] == 0 || (skip && skip[b] != 0)) return;   // gate passed, or map b taken by another kernel
     __shared__ float4 tile[kTile];
     const int o = off[b];
     map_body<kGauss>(d, chg + o, off[b + 1] - o, c2m1, gk, pref, tx_n, blockIdx.z - b * d.Z,

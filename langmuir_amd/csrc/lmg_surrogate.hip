@@ -34,6 +34,7 @@
 #include "langmuir_gpu.h"
 #include "lmg_device.h"
 #include "lmg_gemm.h"
+#include "lmg_mapcull.h"
 #include "lmg_pairsum.h"
 
 namespace {
@@ -424,9 +425,15 @@ struct lmg_surrogate {
     long long chg_cap = 0;
     int* off = nullptr;            // device: list offsets of the batch (max_batch + 1)
     std::vector<int> off_host;     // what off holds
+    std::vector<int32_t> mode_host;   // what map_mode holds
     unsigned int* gate_counter = nullptr;   // per map: blocks finished in gate_reduce_kernel (0 between calls)
     double* gate_err = nullptr;    // device, per map: latest gate error
     int32_t* gate_run = nullptr;   // device, per map: latest decision (1 = exact map used)
+    // the culled exact map of the fallback, for maps where the context's map
+    // would cull: brick counts (zero between calls), offsets, per-charge arrays
+    lmg::MapBricks mc{};           // counts and offsets for max_batch maps
+    long long mc_cap = 0;          // charges the per-charge arrays hold
+    int32_t* map_mode = nullptr;   // device, per map: 1 = culled exact map, 0 = all pairs (from off_host)
     // the gated map as a replayable graph, valid while its key holds
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -770,6 +777,13 @@ lmg_status lmg_surrogate_create(lmg_ctx* ctx, int32_t flags, int32_t max_batch, 
         (st = salloc(s, (void**)&s->gate_run, size_t(max_batch) * sizeof(int32_t))) ||
         (st = salloc(s, (void**)&s->off, size_t(max_batch + 1) * sizeof(int))))
         return fail_create(st);
+    s->mc = lmg::map_bricks_shape(s->dims);
+    if ((st = salloc(s, (void**)&s->mc.cnt, size_t(s->mc.nb) * max_batch * 4)) ||
+        (st = salloc(s, (void**)&s->mc.start, size_t(s->mc.nb + 1) * max_batch * 4)) ||
+        (st = salloc(s, (void**)&s->map_mode, size_t(max_batch) * 4)))
+        return fail_create(st);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for((long long)s->mc.nb * max_batch)), dim3(256), 0, s->stream,
+                       reinterpret_cast<uint32_t*>(s->mc.cnt), (long long)s->mc.nb * max_batch, 0u);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
                        reinterpret_cast<uint32_t*>(s->w), s->nw * 2, 0u);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(s->nw * 2)), dim3(256), 0, s->stream,
@@ -990,6 +1004,11 @@ lmg_status lmg_surrogate_train_dev(lmg_surrogate* s, const int8_t* d_occ, const 
 
 namespace {
 
+// whether the exact map of a list of n charges culls, as the context's own map decides
+bool map_culled(const lmg_surrogate* s, int n) {
+    return !(s->flags & LMG_NO_CUTOFF) && lmg::map_cull_pays(s->dims, s->p.cutoff, n);
+}
+
 // Every launch of a batch of gated maps, on s->stream (also what a graph
 // captures): occupancy grids and charge coordinates, the forward pass, the
 // exact pair sums at the gate sites for every map, the gates, and the exact
@@ -1019,18 +1038,31 @@ void enqueue_gated(lmg_surrogate* s, const int32_t* d_site, const int32_t* d_q, 
     hipLaunchKernelGGL(gate_reduce_kernel, dim3((gate_sites + per - 1) / per, batch), dim3(256), 0, st, s->gate_part,
                        gate_sites, S, s->p.prefactor, d_map, s->V, s->gate_site, s->gate_bpart, s->gate_counter,
                        tolerance, s->gate_err, s->gate_run);
-    // exact maps over the surrogate's where the gate failed (decided on the device)
+    // exact maps over the surrogate's where the gate failed (decided on the device),
+    // each by the kernel the context's own map uses for that list: culled where
+    // it pays (map_mode, uploaded with the offsets), else every charge at every site
+    int culled = 0, max_n = 0;
+    for (int b = 0; b < batch; ++b) {
+        const int n = s->off_host[b + 1] - s->off_host[b];
+        culled += map_culled(s, n) ? 1 : 0;
+        max_n = std::max(max_n, n);
+    }
+    const float pref = float(s->p.prefactor);
+    if (culled > 0)
+        lmg::enqueue_map_cull(st, s->dims, s->mc, s->p.cutoff, gk, pref, gauss, d_site, d_q, max_n, batch,
+                              lmg::MapBatch{s->off, s->gate_run, s->map_mode}, d_map);
+    if (culled == batch) return;
     int tx = 1;
     while (tx < 64 && tx * lmg::kMapT < s->nx) tx <<= 1;
     const int ty = lmg::kBlock / tx;
     const dim3 mgrid((s->nx + tx * lmg::kMapT - 1) / (tx * lmg::kMapT), (s->ny + ty - 1) / ty, s->nz * batch);
-    const float pref = float(s->p.prefactor);
+    const int32_t* skip = culled > 0 ? s->map_mode : nullptr;
     if (gauss)
         hipLaunchKernelGGL(lmg::map_batch_kernel<true>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, s->off, c2m1,
-                           gk, pref, tx, s->gate_run, d_map);
+                           gk, pref, tx, s->gate_run, skip, d_map);
     else
         hipLaunchKernelGGL(lmg::map_batch_kernel<false>, mgrid, dim3(lmg::kBlock), 0, st, s->dims, s->chg, s->off,
-                           c2m1, gk, pref, tx, s->gate_run, d_map);
+                           c2m1, gk, pref, tx, s->gate_run, skip, d_map);
 }
 
 // The gate's charge split: about 4096 waves over the batch, >= 64 charges per
@@ -1070,6 +1102,15 @@ lmg_status gate_buffers(lmg_surrogate* s, long long total, int gate_sites, uint6
     if (bparts > s->gate_bpart_cap) {
         if ((st = regrow((void**)&s->gate_bpart, size_t(bparts) * sizeof(double))) != LMG_OK) return st;
         s->gate_bpart_cap = bparts;
+    }
+    if (total > s->mc_cap) {
+        const long long cap = std::max(total, 2 * s->mc_cap);
+        if ((st = regrow((void**)&s->mc.slot, size_t(cap) * sizeof(int2))) ||
+            (st = regrow((void**)&s->mc.tmp, size_t(cap) * sizeof(float4))) ||
+            (st = regrow((void**)&s->mc.tmpi, size_t(cap) * sizeof(int))) ||
+            (st = regrow((void**)&s->mc.src, size_t(cap) * sizeof(float4))))
+            return st;
+        s->mc_cap = cap;
     }
     if (total > s->chg_cap) {
         const long long cap = std::max(total, 2 * s->chg_cap);
@@ -1133,7 +1174,11 @@ lmg_status lmg_surrogate_map_batch_dev(lmg_surrogate* s, const int32_t* d_site, 
     if (off != s->off_host) {   // the offsets are read by the kernels (and by replays) from device memory
         SHIP(s, hipStreamSynchronize(s->stream));
         s->off_host = off;
+        s->mode_host.resize(batch);
+        for (int b = 0; b < batch; ++b) s->mode_host[b] = map_culled(s, off[b + 1] - off[b]) ? 1 : 0;
         SHIP(s, hipMemcpyAsync(s->off, s->off_host.data(), (batch + 1) * sizeof(int), hipMemcpyHostToDevice,
+                               s->stream));
+        SHIP(s, hipMemcpyAsync(s->map_mode, s->mode_host.data(), batch * sizeof(int32_t), hipMemcpyHostToDevice,
                                s->stream));
     }
     if (!s->use_graphs) {
