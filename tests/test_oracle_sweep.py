@@ -21,6 +21,9 @@ reference rule on every seed.
 * Recombination (drainagent.cpp:113-214): electrons in list order; the candidate holes
   are the same site, then the electron's neighbour list; no draw for one candidate; a
   hole already removed ends the try with no attempt and no draw.
+* balanceCharges (simulation.cpp:253-317): after the exciton source, electrons are injected
+  while they are fewer than the holes; at equal voltages a uniform_01 draw above 0.5 picks
+  the left source, else the right.
 * Range-2 neighbour order: a literal transcription of cubicgrid.cpp:246-437.
 * Placement order (world.cpp:624-643; 659-958; potential.cpp:74-244): defects,
   electrons, holes, then traps (seeds, then growth from a seed's range-1 neighbour
@@ -368,6 +371,75 @@ def scenario_recombination(seed, mistake, oracle=None):
 
 def test_recombination_order(oracle):
     check(oracle, scenario_recombination, ["removed hole not checked", "recombination before decisions"])
+
+
+# ----------------------------------------------------------------------------- scenario E
+# 6x3x1 solar cell, Coulomb off, balance.charges on at equal voltages (simulation.cpp:233-317):
+# one electron at (2,1) = 8, holes at (3,1) = 9 and (4,2) = 16 (neighbours W, E, S; no N on
+# the last row).  Generation rate 0: the exciton source counts an attempt and draws its site,
+# which is then invalid.  Balance: while electrons < holes, a uniform_01 draw above 0.5 picks
+# the left electron source, else the right; the source counts an attempt, draws a face site
+# (x = 0 -> [0, 6, 12], x = 5 -> [5, 11, 17]) and accepts with u < 0.7.
+def scenario_balance(seed, mistake, oracle=None):
+    W = Words(seed)
+    e0, h0, h1 = Carrier(8, [7, 9, 2, 14]), Carrier(9, [8, 10, 3, 15]), Carrier(16, [15, 17, 10])
+    att, suc = [0] * 10, [0] * 10
+    for c in (e0, h0, h1):
+        c.fsite = c.nbrs[W.integer(0, len(c.nbrs) - 1)]
+    for c in (e0, h0, h1):        # every target is Empty in its own grid: each decision draws
+        c.life += 1
+        if W.u01() < 1.0 / 3.0:
+            c.path += 1
+        else:
+            c.fsite = c.site
+    e0.site = e0.fsite
+    taken = {h0.site, h1.site}
+    for c in (h0, h1):            # list order; a site taken earlier in the tick aborts the move
+        if c.fsite != c.site and c.fsite not in taken:
+            taken.discard(c.site)
+            taken.add(c.fsite)
+            c.site = c.fsite
+    injected = []
+
+    def exciton():
+        att[XS] += 1
+        W.integer(0, 17)          # rate 0: the site is drawn, then invalid
+
+    def balance():
+        tries = 0
+        while 1 + len(injected) < 2:
+            coin = W.u01()
+            if mistake == "always left":
+                f = ESL
+            elif mistake == "coin inverted":
+                f = ESL if coin < 0.5 else ESR
+            else:
+                f = ESL if coin > 0.5 else ESR
+            att[f] += 1
+            site = ([0, 6, 12] if f == ESL else [5, 11, 17])[W.integer(0, 2)]
+            if site not in injected and W.u01() < 0.7:
+                suc[f] += 1
+                injected.append(site)
+            tries += 1
+            assert tries <= 1024
+
+    if mistake == "balance first":
+        balance(), exciton()
+    elif mistake == "no balance":
+        exciton()
+    else:
+        exciton(), balance()
+    out = result(W, [e0], [h0, h1], att, suc, injected_e=injected)
+    if oracle is None:
+        return out
+    return make_world(oracle, seed, electrons=[8], holes=[9, 16], simulation_type=1, grid_x=6, grid_y=3,
+                      electron_percentage=0.2, hole_percentage=0.2, generation_rate=0.0, source_scale_area=0.0,
+                      balance_charges=1, e_source_l_rate=0.7, e_source_r_rate=0.7,
+                      e_drain_l_rate=-1.0, e_drain_r_rate=-1.0, h_drain_l_rate=-1.0, h_drain_r_rate=-1.0), out
+
+
+def test_balance_charges_order(oracle):
+    check(oracle, scenario_balance, ["no balance", "always left", "coin inverted", "balance first"])
 
 
 # ----------------------------------------------------------------------------- range 2
