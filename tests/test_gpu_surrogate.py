@@ -177,7 +177,8 @@ def test_batched_gated_maps_match_single_calls(gpu, oracle, dims, cutoff):
     assert np.array_equal(e2, errs) and np.array_equal(u2, used)
 
 
-def test_fallback_takes_the_culled_map_where_the_context_would(gpu, oracle):
+@pytest.mark.parametrize("gauss", [False, True])
+def test_fallback_takes_the_culled_map_where_the_context_would(gpu, oracle, gauss):
     """A failed gate returns the map the context's own lmg_potential_map gives,
     bit for bit, whichever kernel that call picks: 64x64x16 at cutoff 10 with
     2,000 charges culls (the cutoff box of a tile covers a quarter of the lattice),
@@ -187,18 +188,21 @@ def test_fallback_takes_the_culled_map_where_the_context_would(gpu, oracle):
     dims, cutoff = (64, 64, 16), 10
     rng = np.random.default_rng(23)
     V = int(np.prod(dims))
-    ctx, sur = make(gpu, dims, cutoff, max_batch=3)
+    sigma = 1.5 if gauss else 0.0
+    fl = gpu.LMG_GAUSS if gauss else 0
+    ctx, sur = make(gpu, dims, cutoff, flags=fl, sigma=sigma, max_batch=3)
     lists = [random_charges(rng, V, 1000), random_charges(rng, V, 100), (np.zeros(0, np.int32), np.zeros(0, np.int32))]
-    exact = [ctx.potential_map(s, q) for s, q in lists]
+    exact = [ctx.potential_map(s, q, flags=fl) for s, q in lists]
     # the default decision for the long list is the culled kernel, for the short one all pairs
-    assert np.array_equal(exact[0], ctx.potential_map(*lists[0], flags=gpu.LMG_MAP_CULLED))
-    assert not np.array_equal(exact[0], ctx.potential_map(*lists[0], flags=gpu.LMG_MAP_PAIRS))
-    assert np.array_equal(exact[1], ctx.potential_map(*lists[1], flags=gpu.LMG_MAP_PAIRS))
+    assert np.array_equal(exact[0], ctx.potential_map(*lists[0], flags=fl | gpu.LMG_MAP_CULLED))
+    assert not np.array_equal(exact[0], ctx.potential_map(*lists[0], flags=fl | gpu.LMG_MAP_PAIRS))
+    assert np.array_equal(exact[1], ctx.potential_map(*lists[1], flags=fl | gpu.LMG_MAP_PAIRS))
     for b in (0, 1):
-        assert rel_rms(exact[b], oracle.potential_map(dims, cutoff, *lists[b], pref=PREF)) <= 1e-5
+        ref = oracle.potential_map(dims, cutoff, *lists[b], flags=fl, pref=PREF, sigma=sigma)
+        assert rel_rms(exact[b], ref) <= 1e-5
     for sites, q in lists[:2]:   # untrained network: every non-empty list falls back
         out, err, used = sur.map(sites, q, tolerance=1e-3, gate_sites=1024)
-        assert used and np.array_equal(out, ctx.potential_map(sites, q))
+        assert used and np.array_equal(out, ctx.potential_map(sites, q, flags=fl))
     counts = [len(a) for a, _ in lists]
     d_s = gpu.DeviceBuffer.from_array(np.concatenate([a for a, _ in lists]))
     d_q = gpu.DeviceBuffer.from_array(np.concatenate([b for _, b in lists]))
@@ -211,7 +215,7 @@ def test_fallback_takes_the_culled_map_where_the_context_would(gpu, oracle):
         assert used.tolist() == [True, True, False]
         assert np.array_equal(got[0], exact[0]) and np.array_equal(got[1], exact[1])
         assert not got[2].any()
-        ctx.potential_map(*lists[1])   # other work on the stream between replays
+        ctx.potential_map(*lists[1], flags=fl)   # other work on the stream between replays
 
 
 def solar_snapshots(gpu, sur, dims, cutoff, count, stride, seed):
